@@ -1,0 +1,625 @@
+// Device kernels of the single-network NR solver (gfx950, wave64).
+//
+// Stage -> reference kernel it replaces:
+//   k_ybus          host calcYbusSp_Matpower, CPU/esparso/spCalcYbus.cpp:4-90
+//   k_pq            calcPeficiente_0based / calcQeficiente_0based, GPU/PQcalc.cuh:177-219
+//   k_mismatch      calcResPLim / calcResQLim, GPU/newtonRaphson.cuh:35-58
+//   k_limits        host chkLimQ / undo_chkLimQ, CPU/limInjReat.cpp:38-131
+//   k_jfill         SpCalcH/L/M/N, GPU/Jacobiano.cuh:252-344
+//   k_factor_*,
+//   k_fwd_*/k_bwd_* cusolverSpDcsrlsvqr, GPU/newtonRaphson.cuh:466-468
+//   k_update        attOlim / attVlim, GPU/newtonRaphson.cuh:76-119
+//   k_dense         cusolverDnDgetrf/getrs, GPU/newtonRaphson.cuh:507-530
+#include "kernels_impl.hpp"
+
+namespace pf {
+
+// ------------------------------------------------------------------- Ybus
+// One thread per CSR slot; the slot's branch terms are summed in branch
+// order and the bus shunt last, with no contraction into FMA: the result is
+// bitwise the same as a sequential host assembly in branch order.
+__global__ void k_ybus(DevNet N, int nnz) {
+#pragma clang fp contract(off)
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    double re = 0.0, im = 0.0;
+    for (int q = N.yc_ptr[e]; q < N.yc_ptr[e + 1]; q++) {
+        int code = N.yc[q];
+        int l = code >> 2, t = code & 3;
+        double r = N.br_r[l], x = N.br_x[l], b = N.br_b[l], tr = N.br_tr[l], ti = N.br_ti[l];
+        double vr, vi;
+        if (t == 0 || t == 3) {
+            double yr, yi;
+            smith_div(1.0, 0.0, r, x, yr, yi);                    // y = 1/z
+            if (t == 0) {
+                double s = 1.0 / (tr * tr + ti * ti);              // 1/|t|^2
+                vr = yr * s;
+                vi = (yi + b / 2.) * s;                            // (y + jb/2)/|t|^2
+            } else {
+                vr = yr;
+                vi = yi + b / 2.;                                   // y + jb/2
+            }
+        } else {
+            double ar, ai;
+            if (t == 1) smith_div(-1.0, 0.0, tr, -ti, ar, ai);    // -1/conj(t)
+            else smith_div(-1.0, 0.0, tr, ti, ar, ai);            // -1/t
+            smith_div(ar, ai, r, x, vr, vi);                       // .../z
+        }
+        re += vr;
+        im += vi;
+    }
+    int k = N.coorow[e];
+    if (k == N.colind[e]) {
+        re += N.gsh[k];
+        im += N.bsh[k];
+    }
+    N.y[e] = make_double2(re, im);
+}
+
+// --------------------------------------------------------------- voltages
+__global__ void k_init_E(DevNet N, DevState S) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N.nB) return;
+    double s, c;
+    sincos(S.th[k], &s, &c);
+    S.E[k] = make_double2(S.V[k] * c, S.V[k] * s);
+}
+
+// --------------------------------------------------------------------- PQ
+// S_k = E_k conj(sum_m Y_km E_m): LPR lanes per bus row, E staged in LDS for
+// a window of buses around the block's rows (bus numbering keeps most
+// neighbours close), gathered from global memory outside the window.
+template <int LPR>
+__global__ void __launch_bounds__(256) k_pq(DevNet N, DevState S, int gate) {
+    constexpr int ROWS = 256 / LPR;
+    constexpr int WIN = 192;
+    __shared__ double2 win[ROWS + 2 * WIN];
+    if (gate && S.ctl->done) return;
+    const int r0 = blockIdx.x * ROWS;
+    const int r1 = min(N.nB, r0 + ROWS);
+    const int lo = max(0, r0 - WIN), hi = min(N.nB, r1 + WIN);
+    for (int i = threadIdx.x; i < hi - lo; i += blockDim.x) win[i] = S.E[lo + i];
+    __syncthreads();
+    const int row = r0 + (int)threadIdx.x / LPR;
+    const int lane = (int)threadIdx.x % LPR;
+    double ar = 0.0, ai = 0.0;
+    if (row < N.nB) {
+        for (int e = N.rowptr[row] + lane; e < N.rowptr[row + 1]; e += LPR) {
+            int m = N.colind[e];
+            double2 Em = (m >= lo && m < hi) ? win[m - lo] : S.E[m];
+            double2 y = N.y[e];
+            ar += y.x * Em.x - y.y * Em.y;
+            ai += y.x * Em.y + y.y * Em.x;
+        }
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, LPR);
+        ai += __shfl_xor(ai, off, LPR);
+    }
+    if (row < N.nB && lane == 0) {
+        double2 Ek = win[row - lo];
+        S.I[row] = make_double2(ar, ai);
+        S.P[row] = Ek.x * ar + Ek.y * ai;
+        S.Q[row] = Ek.y * ar - Ek.x * ai;
+    }
+}
+
+// ---------------------------------------------------------------- limits
+// One workgroup.  mode 0: start of a pass (reference trigger at iteration
+// >= 1: chkLimQ when max|previous dx| < 0.1, else undo_chkLimQ if any bus is
+// limited, CPU/newtonRaphson.cpp:202-211); mode 1: switch taken at
+// convergence (:252-256); mode 2 / 3: unconditional chkLimQ / undo_chkLimQ.
+__global__ void __launch_bounds__(1024) k_limits(DevNet N, DevState S, int mode, int lim) {
+    __shared__ int s_flips;
+    Ctl *ctl = S.ctl;
+    if (mode == 0 && (ctl->done || !lim || ctl->it < 1)) return;
+    if (mode == 1 && !ctl->post) return;
+    bool do_chk;
+    if (mode == 0) do_chk = u64_as_double(ctl->maxDx_bits) < 0.1;
+    else do_chk = (mode == 1 || mode == 2);
+    if (threadIdx.x == 0) s_flips = 0;
+    int any = 0;
+    if (!do_chk) {
+        for (int i = threadIdx.x; i < N.nPV; i += blockDim.x) any |= S.limQ[i];
+    }
+    any = __syncthreads_or(any);
+    if (!do_chk && !any) return;
+    int flips = 0;
+    for (int i = threadIdx.x; i < N.nPV; i += blockDim.x) {
+        const int b = N.pv_bus[i];
+        const double qload = S.Qload[b];
+        const double qmin = N.qmin[i], qmax = N.qmax[i];
+        unsigned char lq = S.limQ[i];
+        double qll = S.QliqLim[b];
+        if (!lq) {
+            if (do_chk) {
+                double qg = S.Q[b] + qload;
+                if (qg < qmin - kFolga) {
+                    qll = qmin - qload;
+                    lq = 1;
+                    flips++;
+                } else if (qg > qmax + kFolga) {
+                    qll = qmax - qload;
+                    lq = 1;
+                    flips++;
+                }
+            }
+        } else {
+            if (qll == qmax - qload) {
+                if (S.V[b] > N.vset[i] + kFolga) {
+                    lq = 0;
+                    qll = S.Qliq[b];
+                    flips++;
+                }
+            } else {
+                if (S.V[b] < N.vset[i] - kFolga) {
+                    lq = 0;
+                    qll = S.Qliq[b];
+                    flips++;
+                }
+            }
+        }
+        S.limQ[i] = lq;
+        S.QliqLim[b] = qll;
+        S.isq[b] = lq;
+    }
+    if (flips) atomicAdd(&s_flips, flips);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_flips) atomicAdd(&ctl->flips, s_flips);
+}
+
+// --------------------------------------------------------------- mismatch
+__global__ void __launch_bounds__(256) k_mismatch(DevNet N, DevState S, int gate) {
+    if (gate && S.ctl->done) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    unsigned bad = 0;
+    if (k < N.nB && k != N.slack) {
+        int j = N.node_of_bus[k];
+        double gp = S.Pliq[k] - S.P[k];
+        double gq = S.isq[k] ? S.QliqLim[k] - S.Q[k] : 0.0;
+        S.g[2 * j] = gp;
+        S.g[2 * j + 1] = gq;
+        m = dmax_nan(fabs(gp), fabs(gq));
+        int i = N.pv_of_bus[k];
+        if (i >= 0) {
+            double qg = S.Q[k] + S.Qload[k];
+            bad = (qg < N.qmin[i] - kFolga) || (qg > N.qmax[i] + kFolga);
+        }
+    }
+    m = wave_max(m);
+    bad = __any(bad);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&S.ctl->maxG_bits, double_as_u64(m));
+        if (bad) atomicOr(&S.ctl->simple_bad, 1u);
+    }
+}
+
+// Decision of a pass (CPU/newtonRaphson.cpp:247-264): converged, switch at
+// convergence (zero step, as the reference's step there is O(tol)), or step.
+__global__ void k_control(DevState S, int lim) {
+    if (threadIdx.x != 0) return;
+    Ctl *c = S.ctl;
+    if (c->done) return;
+    double maxG = u64_as_double(c->maxG_bits);
+    c->maxG = maxG;
+    if (maxG <= c->tol) {
+        if (lim && c->simple_bad) {
+            c->post = 1;
+            c->step = 0;
+            c->it++;
+        } else {
+            c->done = 1;
+            c->step = 0;
+            c->post = 0;
+        }
+    } else {
+        c->step = 1;
+        c->post = 0;
+        c->it++;
+    }
+    c->maxG_bits = 0;
+    c->simple_bad = 0;
+    c->maxDx_bits = 0;
+}
+
+// ----------------------------------------------------------- Jacobian fill
+// One thread per factor slot.  With c = E_k conj(Y_km E_m), k != m:
+//   H = Im c, N = Re c / V_m, M = -Re c, L = Im c / V_m
+// and on the diagonal (CPU/esparso/spCalcJac.cpp:426-547):
+//   H = -Q - V^2 B, N = (P + V^2 G)/V, M = P - V^2 G, L = (Q - V^2 B)/V.
+// Rows of buses without a Q equation become dV = 0; their dV columns are 0.
+template <typename T>
+__global__ void __launch_bounds__(256) k_jfill(DevNet N, DevState S, DevSym Y, T *blk, int gate) {
+    if (gate && !S.ctl->step) return;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Y.nslot) return;
+    int e = Y.slot_e[s];
+    B4<T> out;
+    if (e < 0) {
+        out = B4<T>{0, 0, 0, 0};
+    } else {
+        int k = N.coorow[e], m = N.colind[e];
+        double2 y = N.y[e];
+        double H, Nn, M, L;
+        if (k != m) {
+            double2 Em = S.E[m], Ek = S.E[k];
+            double tr = y.x * Em.x - y.y * Em.y, ti = y.x * Em.y + y.y * Em.x;   // Y_km E_m
+            double cr = Ek.x * tr + Ek.y * ti, ci = Ek.y * tr - Ek.x * ti;       // E_k conj(.)
+            double vm = S.V[m];
+            H = ci;
+            Nn = cr / vm;
+            M = -cr;
+            L = ci / vm;
+        } else {
+            double v = S.V[k], v2 = v * v, p = S.P[k], q = S.Q[k];
+            H = -q - v2 * y.y;
+            Nn = (p + v2 * y.x) / v;
+            M = p - v2 * y.x;
+            L = (q - v2 * y.y) / v;
+        }
+        if (!S.isq[m]) { Nn = 0; L = 0; }
+        if (!S.isq[k]) { M = 0; L = (k == m) ? 1.0 : 0.0; }
+        out = B4<T>{(T)H, (T)Nn, (T)M, (T)L};
+    }
+    store_b4(blk, s, out);
+}
+
+// ------------------------------------------------------------ block LU
+// Left-looking 2x2-block LU without pivoting over the symbolic program:
+// node j gathers every update from its descendants, so the nodes of one
+// elimination-tree level are independent.  One wavefront per node, one lane
+// per destination block (diagonal first).
+template <typename T>
+__device__ __forceinline__ void factor_node(const DevSym &Y, T *blk, int j, int lane, int *err) {
+    const int d0 = Y.fd_ptr[j], nd = Y.fd_ptr[j + 1] - d0;
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL;
+    B4<T> dinv{0, 0, 0, 0};
+    for (int base = 0; base < nd; base += 64) {
+        const int t = base + lane;
+        B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+        int p = -1;
+        if (t < nd) {
+            const int d = d0 + t;
+            if (t == 0) {
+                a = load_b4(blk, j);
+                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++)
+                    mms(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
+            } else {
+                p = Y.colptr[j] + t - 1;
+                a = load_b4(blk, L0 + p);
+                u = load_b4(blk, U0 + p);
+                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++) {
+                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
+                    mms(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
+                    mms(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
+                }
+            }
+        }
+        if (base == 0) {
+            B4<T> di{0, 0, 0, 0};
+            if (lane == 0) {
+                T det = a.a * a.d - a.b * a.c;
+                if (!(det != (T)0) || !isfinite((double)det)) {
+                    atomicOr(err, 1);
+                    det = (T)1;
+                }
+                di = B4<T>{a.d / det, -a.b / det, -a.c / det, a.a / det};
+                store_b4(blk, j, di);
+            }
+            dinv.a = __shfl(di.a, 0);
+            dinv.b = __shfl(di.b, 0);
+            dinv.c = __shfl(di.c, 0);
+            dinv.d = __shfl(di.d, 0);
+        }
+        if (t < nd && t > 0) {
+            store_b4(blk, L0 + p, mul(a, dinv));
+            store_b4(blk, U0 + p, u);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl, int l0, int count, int gate) {
+    if (gate && !ctl->step) return;
+    const int idx = blockIdx.x * 4 + (int)threadIdx.x / 64;
+    if (idx >= count) return;
+    factor_node(Y, blk, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 63, &ctl->singular);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate) {
+    if (gate && !ctl->step) return;
+    const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
+    for (int l = lv0; l < lv1; l++) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        for (int idx = w; idx < c; idx += 16) factor_node(Y, blk, Y.lvl_nodes[s + idx], lane, &ctl->singular);
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------- triangular solves
+// 16 lanes per node; x holds the right-hand side and is overwritten.
+template <typename T>
+__device__ __forceinline__ void fwd_node(const DevSym &Y, const T *blk, double *x, int j, int sl) {
+    double a0 = 0, a1 = 0;
+    for (int q = Y.rs_ptr[j] + sl; q < Y.rs_ptr[j + 1]; q += 16) {
+        B4<T> L = load_b4(blk, Y.n + Y.rs_p[q]);
+        const int k = Y.rs_k[q];
+        const double x0 = x[2 * k], x1 = x[2 * k + 1];
+        a0 += (double)L.a * x0 + (double)L.b * x1;
+        a1 += (double)L.c * x0 + (double)L.d * x1;
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 16);
+        a1 += __shfl_xor(a1, off, 16);
+    }
+    if (sl == 0) {
+        x[2 * j] -= a0;
+        x[2 * j + 1] -= a1;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bwd_node(const DevSym &Y, const T *blk, double *x, int j, int sl) {
+    double a0 = 0, a1 = 0;
+    const int U0 = Y.n + Y.nnzL;
+    for (int p = Y.colptr[j] + sl; p < Y.colptr[j + 1]; p += 16) {
+        B4<T> U = load_b4(blk, U0 + p);
+        const int i = Y.rowind[p];
+        const double x0 = x[2 * i], x1 = x[2 * i + 1];
+        a0 += (double)U.a * x0 + (double)U.b * x1;
+        a1 += (double)U.c * x0 + (double)U.d * x1;
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 16);
+        a1 += __shfl_xor(a1, off, 16);
+    }
+    if (sl == 0) {
+        const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
+        B4<T> D = load_b4(blk, j);
+        x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
+        x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_level(DevSym Y, const T *blk, double *x, Ctl *ctl, int l0, int count, int gate) {
+    if (gate && !ctl->step) return;
+    const int idx = blockIdx.x * 16 + (int)threadIdx.x / 16;
+    if (idx >= count) return;
+    fwd_node(Y, blk, x, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 15);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_level(DevSym Y, const T *blk, double *x, Ctl *ctl, int l0, int count, int gate) {
+    if (gate && !ctl->step) return;
+    const int idx = blockIdx.x * 16 + (int)threadIdx.x / 16;
+    if (idx >= count) return;
+    bwd_node(Y, blk, x, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 15);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_fwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate) {
+    if (gate && !ctl->step) return;
+    const int g = (int)threadIdx.x / 16, sl = (int)threadIdx.x & 15;
+    for (int l = lv0; l < lv1; l++) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        for (int idx = g; idx < c; idx += 64) fwd_node(Y, blk, x, Y.lvl_nodes[s + idx], sl);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate) {
+    if (gate && !ctl->step) return;
+    const int g = (int)threadIdx.x / 16, sl = (int)threadIdx.x & 15;
+    for (int l = lv1 - 1; l >= lv0; l--) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        for (int idx = g; idx < c; idx += 64) bwd_node(Y, blk, x, Y.lvl_nodes[s + idx], sl);
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- vectors
+__global__ void k_copy(double *dst, const double *src, int n, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_axpy(double *x, const double *d, int n, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+__global__ void k_fill(double *x, size_t n, double v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// FP64 refinement residual r = g - J x, with J x formed from the complex
+// voltages (dE_m = E_m (dV_m / V_m + j dtheta_m), dS_k = dE_k conj(I_k) +
+// E_k conj(sum_m Y_km dE_m)); a bus without a Q equation contributes dV.
+template <int LPR>
+__global__ void __launch_bounds__(256) k_residual(DevNet N, DevState S, int gate) {
+    if (gate && !S.ctl->step) return;
+    constexpr int ROWS = 256 / LPR;
+    const int row = blockIdx.x * ROWS + (int)threadIdx.x / LPR;
+    const int lane = (int)threadIdx.x % LPR;
+    double ar = 0.0, ai = 0.0;
+    const bool active = row < N.nB && row != N.slack;
+    if (active) {
+        for (int e = N.rowptr[row] + lane; e < N.rowptr[row + 1]; e += LPR) {
+            const int m = N.colind[e];
+            if (m == N.slack) continue;
+            const int jm = N.node_of_bus[m];
+            const double dth = S.x[2 * jm];
+            const double dv = S.isq[m] ? S.x[2 * jm + 1] : 0.0;
+            const double2 Em = S.E[m];
+            const double f = dv / S.V[m];
+            const double dr = Em.x * f - Em.y * dth, di = Em.y * f + Em.x * dth;
+            const double2 y = N.y[e];
+            ar += y.x * dr - y.y * di;
+            ai += y.x * di + y.y * dr;
+        }
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, LPR);
+        ai += __shfl_xor(ai, off, LPR);
+    }
+    if (active && lane == 0) {
+        const int j = N.node_of_bus[row];
+        const double dth = S.x[2 * j];
+        const bool q = S.isq[row];
+        const double dv = q ? S.x[2 * j + 1] : 0.0;
+        const double2 Ek = S.E[row], Ik = S.I[row];
+        const double f = dv / S.V[row];
+        const double dr = Ek.x * f - Ek.y * dth, di = Ek.y * f + Ek.x * dth;   // dE_k
+        // dS = dE conj(I) + E conj(Y dE)
+        const double sr = dr * Ik.x + di * Ik.y + Ek.x * ar + Ek.y * ai;
+        const double si = di * Ik.x - dr * Ik.y + Ek.y * ar - Ek.x * ai;
+        S.r[2 * j] = S.g[2 * j] - sr;
+        S.r[2 * j + 1] = q ? S.g[2 * j + 1] - si : -S.x[2 * j + 1];
+    }
+}
+
+// ----------------------------------------------------------------- update
+__global__ void __launch_bounds__(256) k_update(DevNet N, DevState S, int gate) {
+    if (gate && !S.ctl->step) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (k < N.nB && k != N.slack) {
+        const int j = N.node_of_bus[k];
+        const double dth = S.x[2 * j];
+        double v = S.V[k], th = S.th[k] + dth;
+        m = fabs(dth);
+        if (S.isq[k]) {
+            const double dv = S.x[2 * j + 1];
+            v += dv;
+            m = dmax_nan(m, fabs(dv));
+        }
+        S.V[k] = v;
+        S.th[k] = th;
+        double s, c;
+        sincos(th, &s, &c);
+        S.E[k] = make_double2(v * c, v * s);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax(&S.ctl->maxDx_bits, double_as_u64(m));
+}
+
+// ------------------------------------------------------------ dense path
+// metodo denso: the block Jacobian scattered into a dense column-major
+// matrix (node order, 2n x 2n) and factored by one workgroup with partial
+// pivoting; the right-hand side is eliminated alongside.
+__global__ void k_dense_scatter(DevSym Y, const double *blk, double *A, int ld, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Y.nslot) return;
+    int r, c;
+    if (s < Y.n) {
+        r = c = s;
+    } else if (s < Y.n + Y.nnzL) {
+        int p = s - Y.n;
+        r = Y.rowind[p];
+        c = Y.pcol[p];
+    } else {
+        int p = s - Y.n - Y.nnzL;
+        r = Y.pcol[p];
+        c = Y.rowind[p];
+    }
+    B4<double> v = load_b4(blk, s);
+    A[(size_t)(2 * c) * ld + 2 * r] = v.a;
+    A[(size_t)(2 * c + 1) * ld + 2 * r] = v.b;
+    A[(size_t)(2 * c) * ld + 2 * r + 1] = v.c;
+    A[(size_t)(2 * c + 1) * ld + 2 * r + 1] = v.d;
+}
+
+__global__ void __launch_bounds__(1024) k_dense(double *A, double *b, int n, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    __shared__ double sv[16];
+    __shared__ int si[16];
+    __shared__ int s_piv;
+    const int tid = threadIdx.x, w = tid / 64, lane = tid & 63;
+    for (int k = 0; k < n; k++) {
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + tid; i < n; i += 1024) {
+            double a = fabs(A[(size_t)k * n + i]);
+            if (a > best) { best = a; bi = i; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            double ob = __shfl_xor(best, off);
+            int oi = __shfl_xor(bi, off);
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        if (lane == 0) { sv[w] = best; si[w] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            double bb = sv[0];
+            int ii = si[0];
+            for (int q = 1; q < 16; q++)
+                if (sv[q] > bb || (sv[q] == bb && si[q] < ii)) { bb = sv[q]; ii = si[q]; }
+            s_piv = (bb > 0.0 && isfinite(bb)) ? ii : -1;
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (p < 0) {
+            if (tid == 0) atomicOr(&ctl->singular, 1);
+            return;   // uniform: every thread read the same s_piv
+        }
+        if (p != k) {
+            for (int c = tid; c < n; c += 1024) {
+                double t = A[(size_t)c * n + k];
+                A[(size_t)c * n + k] = A[(size_t)c * n + p];
+                A[(size_t)c * n + p] = t;
+            }
+            if (tid == 0) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+        }
+        __syncthreads();
+        const double piv = A[(size_t)k * n + k];
+        for (int i = k + 1 + tid; i < n; i += 1024) A[(size_t)k * n + i] /= piv;
+        __syncthreads();
+        const double bk = b[k];
+        for (int i = k + 1 + tid; i < n; i += 1024) {
+            const double li = A[(size_t)k * n + i];
+            for (int c = k + 1; c < n; c++) A[(size_t)c * n + i] -= li * A[(size_t)c * n + k];
+            b[i] -= li * bk;
+        }
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; k--) {
+        if (tid == 0) b[k] /= A[(size_t)k * n + k];
+        __syncthreads();
+        const double bk = b[k];
+        for (int i = tid; i < k; i += 1024) b[i] -= A[(size_t)k * n + i] * bk;
+        __syncthreads();
+    }
+}
+
+// explicit instantiations
+template __global__ void k_pq<4>(DevNet, DevState, int);
+template __global__ void k_residual<4>(DevNet, DevState, int);
+template __global__ void k_jfill<double>(DevNet, DevState, DevSym, double *, int);
+template __global__ void k_jfill<float>(DevNet, DevState, DevSym, float *, int);
+template __global__ void k_factor_level<double>(DevSym, double *, Ctl *, int, int, int);
+template __global__ void k_factor_level<float>(DevSym, float *, Ctl *, int, int, int);
+template __global__ void k_factor_tail<double>(DevSym, double *, Ctl *, int, int, int);
+template __global__ void k_factor_tail<float>(DevSym, float *, Ctl *, int, int, int);
+template __global__ void k_fwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_fwd_level<float>(DevSym, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_level<float>(DevSym, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_fwd_tail<double>(DevSym, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_fwd_tail<float>(DevSym, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_tail<double>(DevSym, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_tail<float>(DevSym, const float *, double *, Ctl *, int, int, int);
+
+}  // namespace pf

@@ -1,0 +1,68 @@
+// Device kernels of libpf (gfx950, wave64).  Shared by the single-network
+// solver (solver.hip) and the stage entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pf {
+
+constexpr double kFolga = 1e-4;   // tol_limInjReat (CPU/global.cpp:11)
+
+// NR control block, device resident.  Accumulators are reset by k_control
+// after being read, so one captured iteration can be replayed many times.
+struct Ctl {
+    unsigned long long maxG_bits;    // max |g| of this iteration (bit pattern of a double >= 0)
+    unsigned long long maxDx_bits;   // max |dx| of the last step
+    unsigned int simple_bad;         // some PV bus outside [Qmin-folga, Qmax+folga]
+    int it;                          // passes completed (steps + zero steps)
+    int done;                        // converged
+    int step;                        // this pass takes a Newton step
+    int post;                        // this pass is a switch taken at convergence (zero step)
+    int flips;                       // PV<->PQ flips so far
+    int singular;                    // a zero/non-finite pivot was met
+    int max_iter;
+    double tol;
+    double maxG;                     // last max |g|
+};
+
+// Topology and per-bus data of one network on the device.
+struct DevNet {
+    int nB, nL, slack, nPV, n;       // n = nB - 1 block unknowns
+    const int *rowptr, *colind, *coorow, *ydiag;     // Ybus CSR
+    const int *yc_ptr, *yc;                          // Ybus assembly contributions
+    const double *br_r, *br_x, *br_b, *br_tr, *br_ti;
+    const double *gsh, *bsh;
+    double2 *y;                                      // Ybus values
+    const int *node_of_bus;                          // elimination position, -1 = slack
+    const int *bus_of_node;
+    const int *pv_bus, *pv_of_bus;                   // PV list / bus -> PV index (-1)
+    const double *qmin, *qmax, *vset;
+};
+
+struct DevState {
+    double *V, *th;
+    double2 *E, *I;                  // E = V e^{j theta}, I = Y E
+    double *P, *Q;
+    const double *Pliq, *Qliq, *Qload;
+    double *QliqLim;
+    unsigned char *isq;              // bus carries a Q equation (PQ, or PV switched to PQ)
+    unsigned char *limQ;             // per PV
+    double *g;                       // mismatch, 2 per node (node order)
+    double *x;                       // solution / step, 2 per node
+    double *r;                       // refinement residual / correction, 2 per node
+    Ctl *ctl;
+};
+
+// Symbolic factor program on the device.
+struct DevSym {
+    int n, nnzL, nslot;
+    const int *colptr, *rowind, *pcol;      // pcol[p] = column of L position p
+    const int *lvl_ptr, *lvl_nodes;
+    const int *fd_ptr, *fc_ptr, *fc_a, *fc_b;
+    const int *rs_ptr, *rs_k, *rs_p;
+    const int *slot_e;
+};
+
+}  // namespace pf

@@ -1,0 +1,83 @@
+// Internal host-side data model of libpf (not part of the C ABI).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pf.h"
+
+// Network as read from a CDF/CDFX file: 0-based structure of arrays, the
+// layout of the reference's sistemaType/barraType/ramoType
+// (CPU/estrutura/sistema.h:23-59, barra.h:5-22, ramo.h:13-28).
+struct pf_network {
+    int nB = 0, nL = 0, nPV = 0, nPQ = 0, slack = -1, n_duplicates = 0;
+    double baseMVA = 0;
+    std::vector<int> id;
+    std::vector<double> V, theta, Pliq, Qliq, Pload, Qload, Pg, Qg, Vbase, gsh, bsh;
+    std::vector<int> pv, pq;                  // ascending bus indices
+    std::vector<double> Qmin, Qmax, Vset;     // per PV
+    std::vector<int> from, to;
+    std::vector<double> r, x, bch, tap_re, tap_im, phi;
+};
+
+namespace pf {
+
+// Ybus pattern (host, once per network): diagonal + both directions of every
+// branch, columns sorted; plus, per branch, the four CSR slots its pi-model
+// terms land in.  Values are assembled on the GPU.
+struct YPattern {
+    int nB = 0, nnz = 0;
+    std::vector<int> rowptr, colind, coorow, diag;
+    std::vector<int> br_ff, br_ft, br_tf, br_tt;   // per branch
+    // deterministic segmented sum: for each CSR slot, the list of
+    // contributions (branch term codes) in a fixed order
+    std::vector<int> contrib_ptr, contrib;        // code = 4*branch + {0 ff,1 ft,2 tf,3 tt}
+};
+
+YPattern build_ypattern(const pf_network &net);
+
+// Symbolic analysis of the 2x2-block Jacobian on the bus graph minus the
+// slack.  Node j (0..n-1) is a block unknown in elimination order.
+struct Symbolic {
+    int n = 0;                  // nB - 1
+    int nnzL = 0;               // strict-lower 2x2 blocks (= strict-upper blocks)
+    std::vector<int> perm;      // perm[j] = block index (bus order minus slack) of node j
+    std::vector<int> iperm;     // inverse
+    std::vector<int> colptr, rowind;   // struct(L(:,j)), ascending rows > j
+    std::vector<int> parent, level;
+    int nlev = 0;
+    std::vector<int> lvl_ptr, lvl_nodes;   // nodes grouped by level
+    int tail_start = 0;         // first level processed by the single-workgroup tail
+    // factor program: per node, destinations (diag first, then column entries)
+    std::vector<int> fd_ptr;    // [n+1] into destination list
+    std::vector<int> fc_ptr;    // [ndest+1] into contribution pairs
+    std::vector<int> fc_a, fc_b;    // pairs (p_ik, p_jk): positions in L/U arrays
+    // forward-solve rows: per node j, k in rowstruct(j) and position p_jk
+    std::vector<int> rs_ptr, rs_k, rs_p;
+    // Jacobian slot map: block slot -> Y entry (-1: fill-in, zero)
+    //  slots [0, n): diagonal, [n, n+nnzL): L, [n+nnzL, n+2nnzL): U
+    std::vector<int> slot_e;
+    int nslot() const { return n + 2 * nnzL; }
+};
+
+Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width);
+
+// Minimum-degree ordering of a symmetric pattern (CSR, diagonal ignored).
+std::vector<int> min_degree(int n, const std::vector<int> &rowptr, const std::vector<int> &colind);
+
+// Host numeric factor/solve over the same symbolic program (metodo hibridoC).
+// blocks: nslot() 2x2 blocks (row-major 4 doubles each), overwritten by
+// the factors (diagonal slots hold the inverse pivots).  Returns false on a
+// zero/non-finite pivot.
+bool host_block_factor(const Symbolic &s, double *blocks);
+void host_block_solve(const Symbolic &s, const double *blocks, double *x /* 2n, node order */);
+
+// data.txt parsing
+void options_default(pf_options &o);
+pf_status options_parse(const std::string &text, int cpu_layout, pf_options &o);
+
+// CDF / CDFX reader
+pf_status read_case(const std::string &path, double v0, double theta0, pf_network &net);
+
+}  // namespace pf

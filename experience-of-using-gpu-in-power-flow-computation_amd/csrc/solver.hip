@@ -1,0 +1,711 @@
+// Single-network NR driver of libpf: device buffers, the per-pass launch
+// sequence (eager or one captured hipGraph), the three metodo variants and
+// the stage entry points.
+//
+// One NR pass (CPU/newtonRaphson.cpp:180-305, GPU/newtonRaphson.cuh:283-560):
+//   k_pq -> [k_limits: trigger] -> k_mismatch -> k_control -> [k_limits: switch
+//   at convergence] -> step: k_jfill -> factor -> solves (-> refinement) -> k_update
+// Every kernel after k_control is gated on the device flag ctl->step, and
+// every kernel before it on ctl->done, so a captured pass can be replayed
+// until the device says it converged; the host polls one small struct.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels_decl.hpp"
+#include "pf_internal.hpp"
+
+using namespace pf;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define HIPCHK(expr)                                                               \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            g_last_error = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+            return PF_DEVICE_ERROR;                                                \
+        }                                                                          \
+    } while (0)
+
+#define PFCHK(expr)                         \
+    do {                                    \
+        pf_status s_ = (expr);              \
+        if (s_ != PF_OK) return s_;         \
+    } while (0)
+
+constexpr int kTailWidth = 16;   // levels this narrow run inside one workgroup
+
+}  // namespace
+
+struct pf_solver {
+    pf_network net;
+    pf_options opt;
+    int device = 0;
+    YPattern yp;
+    Symbolic sym;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    DevNet dn{};
+    DevState ds{};
+    DevSym dy{};
+    double *blk64 = nullptr;
+    float *blk32 = nullptr;
+    double *dense = nullptr;
+    Ctl *ctl = nullptr;
+    Ctl *ctl_host = nullptr;      // pinned, 2 slots
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<int> e2slot;
+    std::vector<double> Pliq, Qliq, Qload;   // current injections (host copy)
+    std::vector<double> hblk, hx;
+
+    template <typename T>
+    pf_status alloc(T **p, size_t n) {
+        void *q = nullptr;
+        HIPCHK(hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)));
+        allocs.push_back(q);
+        *p = (T *)q;
+        return PF_OK;
+    }
+    template <typename T>
+    pf_status upload(T **p, const std::vector<T> &v) {
+        PFCHK(alloc(p, v.size()));
+        if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        return PF_OK;
+    }
+    template <typename T>
+    pf_status put(T *d, const T *h, size_t n) {
+        if (n) HIPCHK(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, stream));
+        return PF_OK;
+    }
+    template <typename T>
+    pf_status get(T *h, const T *d, size_t n) {
+        if (n) HIPCHK(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return PF_OK;
+    }
+
+    ~pf_solver() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void *p : allocs) (void)hipFree(p);
+        if (ctl_host) (void)hipHostFree(ctl_host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int lim() const { return opt.lim_inj_reat ? 1 : 0; }
+    bool mixed() const { return opt.precision == PF_MIXED && opt.metodo == PF_ESPARSO; }
+
+    pf_status build();
+    pf_status reset_state();
+    pf_status set_isq_from_limq(const unsigned char *limQ);
+    pf_status reset_ctl();
+    pf_status enqueue_head(int gate);
+    pf_status enqueue_factor_solve(int gate);
+    template <typename T> pf_status enqueue_lu(T *blk, int gate);
+    template <typename T> pf_status enqueue_solve(const T *blk, double *v, int gate);
+    pf_status enqueue_step(int gate);
+    pf_status solve(pf_result *r);
+    pf_status run_hybrid_step();
+    pf_status ensure_dense();
+};
+
+// ------------------------------------------------------------------ build
+
+pf_status pf_solver::build() {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void **)&ctl_host, 2 * sizeof(Ctl), hipHostMallocDefault));
+    yp = build_ypattern(net);
+    sym = analyze(net, yp, kTailWidth);
+    const int nB = net.nB, n = sym.n, sl = net.slack;
+    // slot of every Y entry (for the reduced-layout Jacobian export)
+    e2slot.assign(yp.nnz, -1);
+    for (int s = 0; s < sym.nslot(); s++)
+        if (sym.slot_e[s] >= 0) e2slot[sym.slot_e[s]] = s;
+    std::vector<int> node_of_bus(nB, -1), bus_of_node(n), pv_of_bus(nB, -1);
+    for (int k = 0; k < nB; k++)
+        if (k != sl) node_of_bus[k] = sym.iperm[k - (k > sl)];
+    for (int j = 0; j < n; j++) {
+        int b = sym.perm[j];
+        bus_of_node[j] = b + (b >= sl ? 1 : 0);
+    }
+    for (int i = 0; i < net.nPV; i++) pv_of_bus[net.pv[i]] = i;
+    std::vector<int> pcol(sym.nnzL);
+    for (int j = 0; j < n; j++)
+        for (int p = sym.colptr[j]; p < sym.colptr[j + 1]; p++) pcol[p] = j;
+
+    dn.nB = nB;
+    dn.nL = net.nL;
+    dn.slack = sl;
+    dn.nPV = net.nPV;
+    dn.n = n;
+    PFCHK(upload((int **)&dn.rowptr, yp.rowptr));
+    PFCHK(upload((int **)&dn.colind, yp.colind));
+    PFCHK(upload((int **)&dn.coorow, yp.coorow));
+    PFCHK(upload((int **)&dn.ydiag, yp.diag));
+    PFCHK(upload((int **)&dn.yc_ptr, yp.contrib_ptr));
+    PFCHK(upload((int **)&dn.yc, yp.contrib));
+    PFCHK(upload((double **)&dn.br_r, net.r));
+    PFCHK(upload((double **)&dn.br_x, net.x));
+    PFCHK(upload((double **)&dn.br_b, net.bch));
+    PFCHK(upload((double **)&dn.br_tr, net.tap_re));
+    PFCHK(upload((double **)&dn.br_ti, net.tap_im));
+    PFCHK(upload((double **)&dn.gsh, net.gsh));
+    PFCHK(upload((double **)&dn.bsh, net.bsh));
+    PFCHK(alloc(&dn.y, yp.nnz));
+    PFCHK(upload((int **)&dn.node_of_bus, node_of_bus));
+    PFCHK(upload((int **)&dn.bus_of_node, bus_of_node));
+    PFCHK(upload((int **)&dn.pv_bus, net.pv));
+    PFCHK(upload((int **)&dn.pv_of_bus, pv_of_bus));
+    PFCHK(upload((double **)&dn.qmin, net.Qmin));
+    PFCHK(upload((double **)&dn.qmax, net.Qmax));
+    PFCHK(upload((double **)&dn.vset, net.Vset));
+
+    PFCHK(alloc(&ds.V, nB));
+    PFCHK(alloc(&ds.th, nB));
+    PFCHK(alloc(&ds.E, nB));
+    PFCHK(alloc(&ds.I, nB));
+    PFCHK(alloc(&ds.P, nB));
+    PFCHK(alloc(&ds.Q, nB));
+    PFCHK(upload((double **)&ds.Pliq, Pliq));
+    PFCHK(upload((double **)&ds.Qliq, Qliq));
+    PFCHK(upload((double **)&ds.Qload, Qload));
+    PFCHK(alloc(&ds.QliqLim, nB));
+    PFCHK(alloc(&ds.isq, nB));
+    PFCHK(alloc(&ds.limQ, net.nPV));
+    PFCHK(alloc(&ds.g, 2 * (size_t)n));
+    PFCHK(alloc(&ds.x, 2 * (size_t)n));
+    PFCHK(alloc(&ds.r, 2 * (size_t)n));
+    PFCHK(alloc(&ctl, 1));
+    ds.ctl = ctl;
+
+    dy.n = n;
+    dy.nnzL = sym.nnzL;
+    dy.nslot = sym.nslot();
+    PFCHK(upload((int **)&dy.colptr, sym.colptr));
+    PFCHK(upload((int **)&dy.rowind, sym.rowind));
+    PFCHK(upload((int **)&dy.pcol, pcol));
+    PFCHK(upload((int **)&dy.lvl_ptr, sym.lvl_ptr));
+    PFCHK(upload((int **)&dy.lvl_nodes, sym.lvl_nodes));
+    PFCHK(upload((int **)&dy.fd_ptr, sym.fd_ptr));
+    PFCHK(upload((int **)&dy.fc_ptr, sym.fc_ptr));
+    PFCHK(upload((int **)&dy.fc_a, sym.fc_a));
+    PFCHK(upload((int **)&dy.fc_b, sym.fc_b));
+    PFCHK(upload((int **)&dy.rs_ptr, sym.rs_ptr));
+    PFCHK(upload((int **)&dy.rs_k, sym.rs_k));
+    PFCHK(upload((int **)&dy.rs_p, sym.rs_p));
+    PFCHK(upload((int **)&dy.slot_e, sym.slot_e));
+    PFCHK(alloc(&blk64, 4 * (size_t)sym.nslot()));
+    if (opt.precision == PF_MIXED) PFCHK(alloc(&blk32, 4 * (size_t)sym.nslot()));
+
+    // Ybus on the device (once per network)
+    k_ybus<<<(yp.nnz + 255) / 256, 256, 0, stream>>>(dn, yp.nnz);
+    HIPCHK(hipGetLastError());
+    PFCHK(reset_state());
+    HIPCHK(hipStreamSynchronize(stream));
+    return PF_OK;
+}
+
+pf_status pf_solver::ensure_dense() {
+    if (dense) return PF_OK;
+    size_t m = 2 * (size_t)sym.n;
+    if (m > 4096) return PF_INVALID_ARG;   // dense path is for systems up to 4096 unknowns
+    return alloc(&dense, m * m);
+}
+
+pf_status pf_solver::set_isq_from_limq(const unsigned char *limQ) {
+    std::vector<unsigned char> isq(net.nB, 0);
+    for (int k : net.pq) isq[k] = 1;
+    for (int i = 0; i < net.nPV; i++) isq[net.pv[i]] = limQ ? limQ[i] : 0;
+    std::vector<unsigned char> lq(net.nPV, 0);
+    if (limQ) std::copy(limQ, limQ + net.nPV, lq.begin());
+    PFCHK(put(ds.isq, isq.data(), isq.size()));
+    PFCHK(put(ds.limQ, lq.data(), lq.size()));
+    HIPCHK(hipStreamSynchronize(stream));
+    return PF_OK;
+}
+
+// flat start (CPU/estrutura/barra.cpp:20-25 + PV/slack fields) and initLim
+// (CPU/limInjReat.cpp:177-207)
+pf_status pf_solver::reset_state() {
+    PFCHK(put(ds.V, net.V.data(), net.nB));
+    PFCHK(put(ds.th, net.theta.data(), net.nB));
+    PFCHK(put(ds.QliqLim, Qliq.data(), net.nB));
+    PFCHK(set_isq_from_limq(nullptr));
+    return PF_OK;
+}
+
+pf_status pf_solver::reset_ctl() {
+    Ctl c{};
+    c.tol = opt.tolerancia;
+    c.max_iter = opt.no_max_iter;
+    ctl_host[0] = c;
+    PFCHK(put(ctl, ctl_host, 1));
+    HIPCHK(hipStreamSynchronize(stream));
+    return PF_OK;
+}
+
+// ------------------------------------------------------- launch sequences
+
+pf_status pf_solver::enqueue_head(int gate) {
+    const int nB = net.nB;
+    k_pq<4><<<(nB + 63) / 64, 256, 0, stream>>>(dn, ds, gate);
+    if (lim()) k_limits<<<1, 1024, 0, stream>>>(dn, ds, 0, 1);
+    k_mismatch<<<(nB + 255) / 256, 256, 0, stream>>>(dn, ds, gate);
+    k_control<<<1, 64, 0, stream>>>(ds, lim());
+    if (lim()) k_limits<<<1, 1024, 0, stream>>>(dn, ds, 1, 1);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+template <typename T>
+pf_status pf_solver::enqueue_lu(T *blk, int gate) {
+    k_jfill<T><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk, gate);
+    for (int l = 0; l < sym.tail_start; l++) {
+        int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
+        k_factor_level<T><<<(c + 3) / 4, 256, 0, stream>>>(dy, blk, ctl, sym.lvl_ptr[l], c, gate);
+    }
+    if (sym.tail_start < sym.nlev)
+        k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev, gate);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+template <typename T>
+pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
+    for (int l = 0; l < sym.tail_start; l++) {
+        int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
+        k_fwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
+    }
+    if (sym.tail_start < sym.nlev) {
+        k_fwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev, gate);
+        k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev, gate);
+    }
+    for (int l = sym.tail_start - 1; l >= 0; l--) {
+        int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
+        k_bwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
+    }
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+// factor at the current state and solve J x = g (x, in node order)
+pf_status pf_solver::enqueue_factor_solve(int gate) {
+    const int m = 2 * sym.n;
+    if (opt.metodo == PF_DENSO) {
+        PFCHK(ensure_dense());
+        k_jfill<double><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk64, gate);
+        size_t mm = (size_t)m * m;
+        k_fill<<<(unsigned)((mm + 255) / 256), 256, 0, stream>>>(dense, mm, 0.0);
+        k_dense_scatter<<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dy, blk64, dense, m, ctl, gate);
+        k_copy<<<(m + 255) / 256, 256, 0, stream>>>(ds.x, ds.g, m, ctl, gate);
+        k_dense<<<1, 1024, 0, stream>>>(dense, ds.x, m, ctl, gate);
+    } else if (mixed()) {
+        PFCHK(enqueue_lu(blk32, gate));
+        k_copy<<<(m + 255) / 256, 256, 0, stream>>>(ds.x, ds.g, m, ctl, gate);
+        PFCHK(enqueue_solve((const float *)blk32, ds.x, gate));
+        for (int r = 0; r < opt.refine_steps; r++) {
+            k_residual<4><<<(net.nB + 63) / 64, 256, 0, stream>>>(dn, ds, gate);
+            PFCHK(enqueue_solve((const float *)blk32, ds.r, gate));
+            k_axpy<<<(m + 255) / 256, 256, 0, stream>>>(ds.x, ds.r, m, ctl, gate);
+        }
+    } else {
+        PFCHK(enqueue_lu(blk64, gate));
+        k_copy<<<(m + 255) / 256, 256, 0, stream>>>(ds.x, ds.g, m, ctl, gate);
+        PFCHK(enqueue_solve((const double *)blk64, ds.x, gate));
+    }
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+pf_status pf_solver::enqueue_step(int gate) {
+    PFCHK(enqueue_factor_solve(gate));
+    k_update<<<(net.nB + 255) / 256, 256, 0, stream>>>(dn, ds, gate);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+// metodo hibridoC: Jacobian on the GPU, factor and solve on the host over the
+// same symbolic program (the reference: PARDISO, GPU/solverCPU.cpp:218-330).
+pf_status pf_solver::run_hybrid_step() {
+    const size_t ns = sym.nslot();
+    k_jfill<double><<<(ns + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk64, 1);
+    HIPCHK(hipGetLastError());
+    hblk.resize(4 * ns);
+    hx.resize(2 * (size_t)sym.n);
+    PFCHK(get(hblk.data(), blk64, hblk.size()));
+    PFCHK(get(hx.data(), ds.g, hx.size()));
+    if (!host_block_factor(sym, hblk.data())) {
+        ctl_host[0].singular = 1;
+        int one = 1;
+        PFCHK(put(&ctl->singular, &one, 1));
+    } else {
+        host_block_solve(sym, hblk.data(), hx.data());
+    }
+    PFCHK(put(ds.x, hx.data(), hx.size()));
+    k_update<<<(net.nB + 255) / 256, 256, 0, stream>>>(dn, ds, 1);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+pf_status pf_solver::solve(pf_result *res) {
+    HIPCHK(hipSetDevice(device));
+    PFCHK(reset_ctl());
+    auto t0 = std::chrono::steady_clock::now();
+    k_init_E<<<(net.nB + 255) / 256, 256, 0, stream>>>(dn, ds);
+    HIPCHK(hipGetLastError());
+    const int maxit = opt.no_max_iter;
+    Ctl c{};
+    if (opt.use_graph && opt.metodo == PF_ESPARSO) {
+        if (!gexec) {
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            pf_status st = enqueue_head(1);
+            if (st == PF_OK) st = enqueue_step(1);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(stream, &g);
+            if (st != PF_OK) return st;
+            HIPCHK(e);
+            graph = g;
+            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        // pipelined polling: replay i+1 is queued before the host waits on the
+        // copy of the control block that follows replay i
+        int launched = 0, checked = 0;
+        while (checked < maxit) {
+            while (launched < maxit && launched < checked + 2) {
+                HIPCHK(hipGraphLaunch(gexec, stream));
+                HIPCHK(hipMemcpyAsync(&ctl_host[launched & 1], ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipEventRecord(ev[launched & 1], stream));
+                launched++;
+            }
+            HIPCHK(hipEventSynchronize(ev[checked & 1]));
+            c = ctl_host[checked & 1];
+            checked++;
+            if (c.done || c.singular) break;
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        c = ctl_host[(checked - 1) & 1];
+        PFCHK(get(&c, ctl, 1));
+    } else {
+        for (int it = 0; it < maxit; it++) {
+            PFCHK(enqueue_head(1));
+            if (opt.metodo == PF_HIBRIDO_C) {
+                PFCHK(get(&c, ctl, 1));
+                if (c.done) break;
+                if (c.step) PFCHK(run_hybrid_step());
+                PFCHK(get(&c, ctl, 1));
+                if (c.singular) break;
+                continue;
+            }
+            PFCHK(enqueue_step(1));
+            PFCHK(get(&c, ctl, 1));
+            if (c.done || c.singular) break;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    res->iterations = c.it;
+    res->converged = c.done && !c.singular;
+    res->singular = c.singular;
+    res->max_mismatch = c.maxG;
+    res->n_switch_events = c.flips;
+    res->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (c.singular) return PF_SINGULAR;
+    return res->converged ? PF_OK : PF_NOT_CONVERGED;
+}
+
+// ===================================================================== ABI
+
+extern "C" {
+
+pf_status pf_solver_create(const pf_network *net, const pf_options *opt, int device, pf_solver **out) {
+    if (!net || !opt || !out) return PF_INVALID_ARG;
+    *out = nullptr;
+    if (net->nB < 2 || net->slack < 0) return PF_INVALID_ARG;
+    if (opt->metodo < PF_ESPARSO || opt->metodo > PF_HIBRIDO_C) return PF_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        g_last_error = "no HIP device";
+        return PF_DEVICE_ERROR;
+    }
+    auto s = std::make_unique<pf_solver>();
+    s->net = *net;
+    s->opt = *opt;
+    s->device = device;
+    s->Pliq = net->Pliq;
+    s->Qliq = net->Qliq;
+    s->Qload = net->Qload;
+    pf_status st = s->build();
+    if (st != PF_OK) return st;
+    *out = s.release();
+    return PF_OK;
+}
+
+void pf_solver_destroy(pf_solver *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    delete s;
+}
+
+pf_status pf_solve(pf_solver *s, pf_result *res) {
+    if (!s || !res) return PF_INVALID_ARG;
+    std::memset(res, 0, sizeof(*res));
+    return s->solve(res);
+}
+
+pf_status pf_solver_reset(pf_solver *s) {
+    if (!s) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    return s->reset_state();
+}
+
+pf_status pf_solver_set_state(pf_solver *s, const double *V, const double *theta) {
+    if (!s || !V || !theta) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    PFCHK(s->put(s->ds.V, V, s->net.nB));
+    PFCHK(s->put(s->ds.th, theta, s->net.nB));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return PF_OK;
+}
+
+pf_status pf_solver_set_injections(pf_solver *s, const double *Pliq, const double *Qliq,
+                                   const double *Qload) {
+    if (!s || !Pliq || !Qliq || !Qload) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    const int nB = s->net.nB;
+    s->Pliq.assign(Pliq, Pliq + nB);
+    s->Qliq.assign(Qliq, Qliq + nB);
+    s->Qload.assign(Qload, Qload + nB);
+    PFCHK(s->put((double *)s->ds.Pliq, Pliq, nB));
+    PFCHK(s->put((double *)s->ds.Qliq, Qliq, nB));
+    PFCHK(s->put((double *)s->ds.Qload, Qload, nB));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return PF_OK;
+}
+
+pf_status pf_solver_get_state(pf_solver *s, double *V, double *theta, unsigned char *limQ) {
+    if (!s) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    if (V) PFCHK(s->get(V, s->ds.V, s->net.nB));
+    if (theta) PFCHK(s->get(theta, s->ds.th, s->net.nB));
+    if (limQ) PFCHK(s->get(limQ, s->ds.limQ, s->net.nPV));
+    return PF_OK;
+}
+
+pf_status pf_solver_lu_info(pf_solver *s, int *n, int *nnzL, int *levels, int *tail_levels) {
+    if (!s) return PF_INVALID_ARG;
+    if (n) *n = s->sym.n;
+    if (nnzL) *nnzL = s->sym.nnzL;
+    if (levels) *levels = s->sym.nlev;
+    if (tail_levels) *tail_levels = s->sym.nlev - s->sym.tail_start;
+    return PF_OK;
+}
+
+const char *pf_last_error(void) { return g_last_error.c_str(); }
+
+// ------------------------------------------------------------ stages
+
+pf_status pf_stage_ybus(pf_solver *s, int *rowptr, int *colind, double *y, int *nnz) {
+    if (!s) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    if (nnz) *nnz = s->yp.nnz;
+    if (rowptr) std::copy(s->yp.rowptr.begin(), s->yp.rowptr.end(), rowptr);
+    if (colind) std::copy(s->yp.colind.begin(), s->yp.colind.end(), colind);
+    if (y) PFCHK(s->get(y, (const double *)s->dn.y, 2 * (size_t)s->yp.nnz));
+    return PF_OK;
+}
+
+static pf_status stage_state(pf_solver *s, const double *V, const double *theta,
+                             const unsigned char *limQ, const double *QliqLim) {
+    HIPCHK(hipSetDevice(s->device));
+    PFCHK(s->reset_ctl());
+    PFCHK(s->put(s->ds.V, V, s->net.nB));
+    PFCHK(s->put(s->ds.th, theta, s->net.nB));
+    PFCHK(s->set_isq_from_limq(limQ));
+    if (QliqLim) PFCHK(s->put(s->ds.QliqLim, QliqLim, s->net.nB));
+    k_init_E<<<(s->net.nB + 255) / 256, 256, 0, s->stream>>>(s->dn, s->ds);
+    k_pq<4><<<(s->net.nB + 63) / 64, 256, 0, s->stream>>>(s->dn, s->ds, 0);
+    HIPCHK(hipGetLastError());
+    return PF_OK;
+}
+
+pf_status pf_stage_pq(pf_solver *s, const double *V, const double *theta, double *P, double *Q) {
+    if (!s || !V || !theta) return PF_INVALID_ARG;
+    PFCHK(stage_state(s, V, theta, nullptr, nullptr));
+    if (P) PFCHK(s->get(P, s->ds.P, s->net.nB));
+    if (Q) PFCHK(s->get(Q, s->ds.Q, s->net.nB));
+    return PF_OK;
+}
+
+// reference layout of the reduced system: rows/cols [non-slack buses][PQlim buses]
+static void reduced_lists(const pf_solver *s, const unsigned char *limQ, std::vector<int> &pqlim) {
+    std::vector<unsigned char> isq(s->net.nB, 0);
+    for (int k : s->net.pq) isq[k] = 1;
+    if (limQ)
+        for (int i = 0; i < s->net.nPV; i++) isq[s->net.pv[i]] = limQ[i];
+    pqlim.clear();
+    for (int k = 0; k < s->net.nB; k++)
+        if (isq[k]) pqlim.push_back(k);
+}
+
+pf_status pf_stage_mismatch(pf_solver *s, const double *V, const double *theta,
+                            const unsigned char *limQ, const double *QliqLim, double *g,
+                            double *max_abs) {
+    if (!s || !V || !theta || !QliqLim) return PF_INVALID_ARG;
+    PFCHK(stage_state(s, V, theta, limQ, QliqLim));
+    k_mismatch<<<(s->net.nB + 255) / 256, 256, 0, s->stream>>>(s->dn, s->ds, 0);
+    HIPCHK(hipGetLastError());
+    std::vector<double> gn(2 * (size_t)s->sym.n);
+    PFCHK(s->get(gn.data(), s->ds.g, gn.size()));
+    Ctl c;
+    PFCHK(s->get(&c, s->ctl, 1));
+    double m;
+    std::memcpy(&m, &c.maxG_bits, sizeof(m));
+    if (max_abs) *max_abs = m;
+    if (g) {
+        std::vector<int> pqlim;
+        reduced_lists(s, limQ, pqlim);
+        const int nB = s->net.nB, sl = s->net.slack;
+        std::vector<int> node(nB, -1);
+        for (int k = 0; k < nB; k++)
+            if (k != sl) node[k] = s->sym.iperm[k - (k > sl)];
+        for (int k = 0; k < nB; k++)
+            if (k != sl) g[k - (k > sl)] = gn[2 * node[k]];
+        for (size_t i = 0; i < pqlim.size(); i++) g[nB - 1 + i] = gn[2 * node[pqlim[i]] + 1];
+    }
+    return PF_OK;
+}
+
+pf_status pf_stage_jacobian(pf_solver *s, const double *V, const double *theta,
+                            const unsigned char *limQ, int *nJ, int *rowptr, int *col,
+                            double *val, int cap, int *nnz) {
+    if (!s || !V || !theta) return PF_INVALID_ARG;
+    PFCHK(stage_state(s, V, theta, limQ, nullptr));
+    const size_t ns = s->sym.nslot();
+    k_jfill<double><<<(ns + 255) / 256, 256, 0, s->stream>>>(s->dn, s->ds, s->dy, s->blk64, 0);
+    HIPCHK(hipGetLastError());
+    std::vector<double> b(4 * ns);
+    PFCHK(s->get(b.data(), s->blk64, b.size()));
+    std::vector<int> pqlim;
+    reduced_lists(s, limQ, pqlim);
+    const int nB = s->net.nB, sl = s->net.slack;
+    std::vector<int> pos(nB, -1);
+    for (size_t i = 0; i < pqlim.size(); i++) pos[pqlim[i]] = (int)i;
+    const auto &yp = s->yp;
+    int count = 0, row = 0;
+    if (rowptr) rowptr[0] = 0;
+    auto push = [&](int c, double v) {
+        if (count < cap && col && val) { col[count] = c; val[count] = v; }
+        count++;
+    };
+    auto block = [&](int e) { return &b[4 * (size_t)s->e2slot[e]]; };
+    for (int k = 0; k < nB; k++) {          // rows dP_k: H then N (CPU/esparso/spCalcJac.cpp:409-504)
+        if (k == sl) continue;
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (m == sl) continue;
+            push(m - (m > sl), block(e)[0]);
+        }
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (pos[m] < 0) continue;
+            push(nB - 1 + pos[m], block(e)[1]);
+        }
+        if (rowptr) rowptr[++row] = count;
+    }
+    for (int k : pqlim) {                    // rows dQ_k: M then L (:506-553)
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (m == sl) continue;
+            push(m - (m > sl), block(e)[2]);
+        }
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (pos[m] < 0) continue;
+            push(nB - 1 + pos[m], block(e)[3]);
+        }
+        if (rowptr) rowptr[++row] = count;
+    }
+    if (nJ) *nJ = nB - 1 + (int)pqlim.size();
+    if (nnz) *nnz = count;
+    return count <= cap ? PF_OK : PF_INVALID_ARG;
+}
+
+pf_status pf_stage_lu_solve(pf_solver *s, const double *V, const double *theta,
+                            const unsigned char *limQ, const double *rhs, double *x) {
+    if (!s || !V || !theta || !rhs || !x) return PF_INVALID_ARG;
+    PFCHK(stage_state(s, V, theta, limQ, nullptr));
+    std::vector<int> pqlim;
+    reduced_lists(s, limQ, pqlim);
+    const int nB = s->net.nB, sl = s->net.slack, n = s->sym.n;
+    std::vector<int> node(nB, -1);
+    for (int k = 0; k < nB; k++)
+        if (k != sl) node[k] = s->sym.iperm[k - (k > sl)];
+    std::vector<double> gn(2 * (size_t)n, 0.0);
+    for (int k = 0; k < nB; k++)
+        if (k != sl) gn[2 * node[k]] = rhs[k - (k > sl)];
+    for (size_t i = 0; i < pqlim.size(); i++) gn[2 * node[pqlim[i]] + 1] = rhs[nB - 1 + i];
+    PFCHK(s->put(s->ds.g, gn.data(), gn.size()));
+    // force the gated kernels of one step to run
+    Ctl c{};
+    c.step = 1;
+    c.tol = s->opt.tolerancia;
+    s->ctl_host[0] = c;
+    PFCHK(s->put(s->ctl, s->ctl_host, 1));
+    std::vector<double> xn(2 * (size_t)n);
+    if (s->opt.metodo == PF_HIBRIDO_C) {
+        const size_t ns = s->sym.nslot();
+        k_jfill<double><<<(ns + 255) / 256, 256, 0, s->stream>>>(s->dn, s->ds, s->dy, s->blk64, 0);
+        HIPCHK(hipGetLastError());
+        std::vector<double> b(4 * ns);
+        PFCHK(s->get(b.data(), s->blk64, b.size()));
+        if (!host_block_factor(s->sym, b.data())) return PF_SINGULAR;
+        xn = gn;
+        host_block_solve(s->sym, b.data(), xn.data());
+    } else {
+        PFCHK(s->enqueue_factor_solve(1));
+        PFCHK(s->get(xn.data(), s->ds.x, xn.size()));
+        PFCHK(s->get(&c, s->ctl, 1));
+        if (c.singular) return PF_SINGULAR;
+    }
+    for (int k = 0; k < nB; k++)
+        if (k != sl) x[k - (k > sl)] = xn[2 * node[k]];
+    for (size_t i = 0; i < pqlim.size(); i++) x[nB - 1 + i] = xn[2 * node[pqlim[i]] + 1];
+    return PF_OK;
+}
+
+pf_status pf_stage_limits(pf_solver *s, int mode, const double *V, const double *Q,
+                          unsigned char *limQ, double *QliqLim, int *flips) {
+    if (!s || !V || !Q || !limQ || !QliqLim || (mode != 0 && mode != 1)) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    PFCHK(s->reset_ctl());
+    PFCHK(s->put(s->ds.V, V, s->net.nB));
+    PFCHK(s->put(s->ds.Q, Q, s->net.nB));
+    PFCHK(s->set_isq_from_limq(limQ));
+    PFCHK(s->put(s->ds.QliqLim, QliqLim, s->net.nB));
+    k_limits<<<1, 1024, 0, s->stream>>>(s->dn, s->ds, mode == 0 ? 2 : 3, 1);
+    HIPCHK(hipGetLastError());
+    PFCHK(s->get(limQ, s->ds.limQ, s->net.nPV));
+    PFCHK(s->get(QliqLim, s->ds.QliqLim, s->net.nB));
+    Ctl c;
+    PFCHK(s->get(&c, s->ctl, 1));
+    if (flips) *flips = c.flips;
+    return PF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,340 @@
+// Host-side symbolic work of libpf, done once per network:
+//  * the Ybus CSR pattern and the per-slot contribution lists that make the
+//    GPU assembly a deterministic segmented sum (no float atomics);
+//  * ordering + symbolic factorisation of the 2x2-block Jacobian on the bus
+//    graph (minus the slack), elimination-tree levels, and the "factor
+//    program" (for every node, every destination block and the list of
+//    source-block pairs that update it);
+//  * the host numeric factor/solve over that program (metodo hibridoC).
+//
+// Why a fixed 2x2-block pattern: the reference rebuilds the Jacobian pattern
+// and redoes the symbolic analysis whenever a PV bus switches to PQ
+// (CPU/esparso/spCalcJac.cpp:402-568; PARDISO phase 11 each iteration,
+// CPU/esparso/spSolver.cpp:141-153).  Here every non-slack bus keeps two
+// unknowns (dtheta, dV) and two equations (dP, dQ); for a PV bus the dQ row
+// is replaced by dV = 0 and its dV column is zeroed, which leaves the
+// reference's reduced system embedded unchanged, so one symbolic analysis
+// serves every switching state.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <queue>
+
+#include "pf_internal.hpp"
+
+namespace pf {
+
+YPattern build_ypattern(const pf_network &net) {
+    struct T {
+        int row, col, code;
+    };
+    std::vector<T> t;
+    t.reserve(net.nB + 4 * (size_t)net.nL);
+    for (int l = 0; l < net.nL; l++) {
+        int f = net.from[l], to = net.to[l];
+        t.push_back({f, f, 4 * l + 0});
+        t.push_back({f, to, 4 * l + 1});
+        t.push_back({to, f, 4 * l + 2});
+        t.push_back({to, to, 4 * l + 3});
+    }
+    for (int i = 0; i < net.nB; i++) t.push_back({i, i, INT32_MAX});   // shunt, last in its slot
+    std::sort(t.begin(), t.end(), [](const T &a, const T &b) {
+        if (a.row != b.row) return a.row < b.row;
+        if (a.col != b.col) return a.col < b.col;
+        return a.code < b.code;
+    });
+    YPattern y;
+    y.nB = net.nB;
+    y.rowptr.assign(net.nB + 1, 0);
+    y.diag.assign(net.nB, -1);
+    y.br_ff.assign(net.nL, -1);
+    y.br_ft.assign(net.nL, -1);
+    y.br_tf.assign(net.nL, -1);
+    y.br_tt.assign(net.nL, -1);
+    y.contrib_ptr.push_back(0);
+    for (size_t i = 0; i < t.size();) {
+        size_t j = i;
+        int slot = (int)y.colind.size();
+        while (j < t.size() && t[j].row == t[i].row && t[j].col == t[i].col) {
+            if (t[j].code != INT32_MAX) {
+                y.contrib.push_back(t[j].code);
+                int l = t[j].code >> 2;
+                switch (t[j].code & 3) {
+                    case 0: y.br_ff[l] = slot; break;
+                    case 1: y.br_ft[l] = slot; break;
+                    case 2: y.br_tf[l] = slot; break;
+                    default: y.br_tt[l] = slot; break;
+                }
+            }
+            j++;
+        }
+        y.colind.push_back(t[i].col);
+        y.coorow.push_back(t[i].row);
+        if (t[i].row == t[i].col) y.diag[t[i].row] = slot;
+        y.rowptr[t[i].row + 1] = (int)y.colind.size();
+        y.contrib_ptr.push_back((int)y.contrib.size());
+        i = j;
+    }
+    for (int r = 0; r < net.nB; r++) y.rowptr[r + 1] = std::max(y.rowptr[r + 1], y.rowptr[r]);
+    y.nnz = (int)y.colind.size();
+    return y;
+}
+
+// Minimum degree on the explicit elimination graph; ties broken by the
+// lowest index, so the ordering is a pure function of the pattern.
+std::vector<int> min_degree(int n, const std::vector<int> &rowptr, const std::vector<int> &colind) {
+    std::vector<std::vector<int>> adj(n);
+    for (int i = 0; i < n; i++)
+        for (int p = rowptr[i]; p < rowptr[i + 1]; p++) {
+            int j = colind[p];
+            if (j == i) continue;
+            adj[i].push_back(j);
+            adj[j].push_back(i);
+        }
+    for (auto &a : adj) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    using E = std::pair<int, int>;   // (degree, node)
+    std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+    for (int i = 0; i < n; i++) pq.push({(int)adj[i].size(), i});
+    std::vector<char> dead(n, 0);
+    std::vector<int> perm;
+    perm.reserve(n);
+    std::vector<int> tmp;
+    while ((int)perm.size() < n) {
+        auto [d, v] = pq.top();
+        pq.pop();
+        if (dead[v] || d != (int)adj[v].size()) continue;
+        dead[v] = 1;
+        perm.push_back(v);
+        const std::vector<int> nv = adj[v];
+        for (int u : nv) {
+            tmp.clear();
+            std::set_union(adj[u].begin(), adj[u].end(), nv.begin(), nv.end(), std::back_inserter(tmp));
+            tmp.erase(std::remove_if(tmp.begin(), tmp.end(), [&](int x) { return x == u || x == v; }),
+                      tmp.end());
+            adj[u].swap(tmp);
+            pq.push({(int)adj[u].size(), u});
+        }
+        adj[v].clear();
+        adj[v].shrink_to_fit();
+    }
+    return perm;
+}
+
+Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width) {
+    Symbolic s;
+    const int nB = net.nB, sl = net.slack;
+    const int n = nB - 1;
+    s.n = n;
+    auto blk = [&](int k) { return k - (k > sl ? 1 : 0); };
+    // bus graph without the slack, in block numbering
+    std::vector<int> gp(n + 1, 0), gi;
+    for (int k = 0; k < nB; k++) {
+        if (k == sl) continue;
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (m == sl || m == k) continue;
+            gi.push_back(blk(m));
+        }
+        gp[blk(k) + 1] = (int)gi.size();
+    }
+    s.perm = min_degree(n, gp, gi);
+    s.iperm.assign(n, 0);
+    for (int j = 0; j < n; j++) s.iperm[s.perm[j]] = j;
+    // upper adjacency in elimination order
+    std::vector<std::vector<int>> up(n);
+    for (int b = 0; b < n; b++)
+        for (int p = gp[b]; p < gp[b + 1]; p++) {
+            int a = s.iperm[b], c = s.iperm[gi[p]];
+            if (a < c) up[a].push_back(c);
+            else if (c < a) up[c].push_back(a);
+        }
+    // column structures by merging children (elimination tree):
+    // struct(j) = (upper adjacency of j  U  struct of every child) \ {<= j}
+    std::vector<std::vector<int>> st(n), children(n);
+    s.parent.assign(n, -1);
+    std::vector<int> tmp;
+    for (int j = 0; j < n; j++) {
+        std::vector<int> cur = up[j];
+        std::sort(cur.begin(), cur.end());
+        cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+        for (int c : children[j]) {
+            tmp.clear();
+            std::set_union(cur.begin(), cur.end(), st[c].begin(), st[c].end(), std::back_inserter(tmp));
+            cur.swap(tmp);
+        }
+        cur.erase(std::remove_if(cur.begin(), cur.end(), [&](int x) { return x <= j; }), cur.end());
+        if (!cur.empty()) {
+            s.parent[j] = cur[0];
+            children[cur[0]].push_back(j);
+        }
+        st[j].swap(cur);
+    }
+    s.colptr.assign(n + 1, 0);
+    for (int j = 0; j < n; j++) s.colptr[j + 1] = s.colptr[j] + (int)st[j].size();
+    s.nnzL = s.colptr[n];
+    s.rowind.resize(s.nnzL);
+    for (int j = 0; j < n; j++) std::copy(st[j].begin(), st[j].end(), s.rowind.begin() + s.colptr[j]);
+    auto pos = [&](int i, int j) -> int {   // position of row i in column j, -1 if absent
+        auto b = s.rowind.begin() + s.colptr[j], e = s.rowind.begin() + s.colptr[j + 1];
+        auto it = std::lower_bound(b, e, i);
+        return (it != e && *it == i) ? (int)(it - s.rowind.begin()) : -1;
+    };
+    // levels: leaves 0, parent above every child
+    s.level.assign(n, 0);
+    for (int j = 0; j < n; j++)
+        if (s.parent[j] >= 0) s.level[s.parent[j]] = std::max(s.level[s.parent[j]], s.level[j] + 1);
+    s.nlev = 0;
+    for (int j = 0; j < n; j++) s.nlev = std::max(s.nlev, s.level[j] + 1);
+    s.lvl_ptr.assign(s.nlev + 1, 0);
+    for (int j = 0; j < n; j++) s.lvl_ptr[s.level[j] + 1]++;
+    for (int l = 0; l < s.nlev; l++) s.lvl_ptr[l + 1] += s.lvl_ptr[l];
+    s.lvl_nodes.resize(n);
+    {
+        std::vector<int> fill(s.lvl_ptr.begin(), s.lvl_ptr.end() - 1);
+        for (int j = 0; j < n; j++) s.lvl_nodes[fill[s.level[j]]++] = j;
+    }
+    s.tail_start = s.nlev;
+    while (s.tail_start > 0 &&
+           s.lvl_ptr[s.tail_start] - s.lvl_ptr[s.tail_start - 1] <= tail_width)
+        s.tail_start--;
+    // row structures (forward solve): for row i, every k < i with i in struct(col k)
+    s.rs_ptr.assign(n + 1, 0);
+    for (int p = 0; p < s.nnzL; p++) s.rs_ptr[s.rowind[p] + 1]++;
+    for (int i = 0; i < n; i++) s.rs_ptr[i + 1] += s.rs_ptr[i];
+    s.rs_k.resize(s.nnzL);
+    s.rs_p.resize(s.nnzL);
+    {
+        std::vector<int> fill(s.rs_ptr.begin(), s.rs_ptr.end() - 1);
+        for (int k = 0; k < n; k++)
+            for (int p = s.colptr[k]; p < s.colptr[k + 1]; p++) {
+                int i = s.rowind[p];
+                s.rs_k[fill[i]] = k;
+                s.rs_p[fill[i]] = p;
+                fill[i]++;
+            }
+    }
+    // factor program
+    s.fd_ptr.assign(n + 1, 0);
+    s.fc_ptr.assign(1, 0);
+    std::vector<int> local(n, -1);
+    std::vector<std::vector<std::pair<int, int>>> dl;
+    for (int j = 0; j < n; j++) {
+        int nd = 1 + (s.colptr[j + 1] - s.colptr[j]);
+        s.fd_ptr[j + 1] = s.fd_ptr[j] + nd;
+        dl.assign(nd, {});
+        for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) local[s.rowind[p]] = 1 + (p - s.colptr[j]);
+        for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) {
+            int k = s.rs_k[q], pjk = s.rs_p[q];
+            dl[0].push_back({pjk, pjk});
+            for (int p = pjk + 1; p < s.colptr[k + 1]; p++) {   // rows of col k below j
+                int i = s.rowind[p];
+                dl[local[i]].push_back({p, pjk});
+            }
+        }
+        for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) local[s.rowind[p]] = -1;
+        for (int d = 0; d < nd; d++) {
+            for (auto &pr : dl[d]) {
+                s.fc_a.push_back(pr.first);
+                s.fc_b.push_back(pr.second);
+            }
+            s.fc_ptr.push_back((int)s.fc_a.size());
+        }
+    }
+    // Jacobian slot map
+    s.slot_e.assign(s.nslot(), -1);
+    for (int k = 0; k < nB; k++) {
+        if (k == sl) continue;
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (m == sl) continue;
+            int a = s.iperm[blk(k)], c = s.iperm[blk(m)];
+            int slot;
+            if (a == c) slot = a;
+            else if (a > c) slot = n + pos(a, c);
+            else slot = n + s.nnzL + pos(c, a);
+            s.slot_e[slot] = e;
+        }
+    }
+    return s;
+}
+
+// ------------------------------------------------------------- host numeric
+
+namespace {
+
+inline void mms(double *c, const double *a, const double *b) {   // c -= a*b (2x2)
+    double c0 = c[0] - (a[0] * b[0] + a[1] * b[2]);
+    double c1 = c[1] - (a[0] * b[1] + a[1] * b[3]);
+    double c2 = c[2] - (a[2] * b[0] + a[3] * b[2]);
+    double c3 = c[3] - (a[2] * b[1] + a[3] * b[3]);
+    c[0] = c0; c[1] = c1; c[2] = c2; c[3] = c3;
+}
+
+}  // namespace
+
+bool host_block_factor(const Symbolic &s, double *blk) {
+    const int n = s.n, L0 = n, U0 = n + s.nnzL;
+    for (int j = 0; j < n; j++) {
+        double dinv[4];
+        for (int d = s.fd_ptr[j]; d < s.fd_ptr[j + 1]; d++) {
+            int t = d - s.fd_ptr[j];
+            if (t == 0) {
+                double *D = blk + 4 * (size_t)j;
+                for (int q = s.fc_ptr[d]; q < s.fc_ptr[d + 1]; q++)
+                    mms(D, blk + 4 * (size_t)(L0 + s.fc_a[q]), blk + 4 * (size_t)(U0 + s.fc_b[q]));
+                double det = D[0] * D[3] - D[1] * D[2];
+                if (!(det != 0.0) || !std::isfinite(det)) return false;
+                dinv[0] = D[3] / det; dinv[1] = -D[1] / det;
+                dinv[2] = -D[2] / det; dinv[3] = D[0] / det;
+                for (int k = 0; k < 4; k++) D[k] = dinv[k];
+            } else {
+                int p = s.colptr[j] + t - 1;
+                double *Lb = blk + 4 * (size_t)(L0 + p), *Ub = blk + 4 * (size_t)(U0 + p);
+                for (int q = s.fc_ptr[d]; q < s.fc_ptr[d + 1]; q++) {
+                    int pik = s.fc_a[q], pjk = s.fc_b[q];
+                    mms(Lb, blk + 4 * (size_t)(L0 + pik), blk + 4 * (size_t)(U0 + pjk));
+                    mms(Ub, blk + 4 * (size_t)(L0 + pjk), blk + 4 * (size_t)(U0 + pik));
+                }
+                double l0 = Lb[0] * dinv[0] + Lb[1] * dinv[2];
+                double l1 = Lb[0] * dinv[1] + Lb[1] * dinv[3];
+                double l2 = Lb[2] * dinv[0] + Lb[3] * dinv[2];
+                double l3 = Lb[2] * dinv[1] + Lb[3] * dinv[3];
+                Lb[0] = l0; Lb[1] = l1; Lb[2] = l2; Lb[3] = l3;
+            }
+        }
+    }
+    return true;
+}
+
+void host_block_solve(const Symbolic &s, const double *blk, double *x) {
+    const int n = s.n, L0 = n, U0 = n + s.nnzL;
+    for (int j = 0; j < n; j++) {
+        double y0 = x[2 * j], y1 = x[2 * j + 1];
+        for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) {
+            const double *L = blk + 4 * (size_t)(L0 + s.rs_p[q]);
+            int k = s.rs_k[q];
+            y0 -= L[0] * x[2 * k] + L[1] * x[2 * k + 1];
+            y1 -= L[2] * x[2 * k] + L[3] * x[2 * k + 1];
+        }
+        x[2 * j] = y0;
+        x[2 * j + 1] = y1;
+    }
+    for (int j = n - 1; j >= 0; j--) {
+        double y0 = x[2 * j], y1 = x[2 * j + 1];
+        for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) {
+            const double *U = blk + 4 * (size_t)(U0 + p);
+            int i = s.rowind[p];
+            y0 -= U[0] * x[2 * i] + U[1] * x[2 * i + 1];
+            y1 -= U[2] * x[2 * i] + U[3] * x[2 * i + 1];
+        }
+        const double *Di = blk + 4 * (size_t)j;
+        x[2 * j] = Di[0] * y0 + Di[1] * y1;
+        x[2 * j + 1] = Di[2] * y0 + Di[3] * y1;
+    }
+}
+
+}  // namespace pf

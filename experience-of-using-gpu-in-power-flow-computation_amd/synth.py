@@ -146,7 +146,8 @@ def _r(x: float, nd: int) -> float:
 
 def _area(seed: int, n_bus: int, n_pv: int, n_chords: int, n_taps: int,
           total_load_pu: float | None, id0: int = 0, pv_share: float = 0.84,
-          load_on_pv: bool = False, qlim_scale: float = 0.0) -> Case:
+          load_on_pv: bool = False, qlim_scale: float = 0.0,
+          root_at_slack: bool = False) -> Case:
     rng = SplitMix64(seed)
     base = 100.0
     # bus 1 (local index 0) is the slack; PV buses drawn from the rest
@@ -183,9 +184,13 @@ def _area(seed: int, n_bus: int, n_pv: int, n_chords: int, n_taps: int,
         b.qmin = _r(-rng.uniform(0.2, 0.5) * qs * base, 2)
         b.pg = _r(pv_total * wi / wsum, 2)
     buses[0].pg = _r(1.05 * ptot_mw - sum(b.pg for b in pvs), 2)
-    # topology: random recursive spanning tree rooted at the slack (bus 1 of
-    # the area, so the slack is the best-connected bus) + distinct chords
-    order = [0] + [1 + p for p in rng.permutation(n_bus - 1)]
+    # topology: random recursive spanning tree (rooted at the slack, bus 1
+    # of the area, when root_at_slack: the slack is then the best-connected
+    # bus) + distinct chords
+    if root_at_slack:
+        order = [0] + [1 + p for p in rng.permutation(n_bus - 1)]
+    else:
+        order = rng.permutation(n_bus)
     pairs = []
     seen = set()
     for k in range(1, n_bus):
@@ -351,7 +356,7 @@ def area118(seed: int, id0: int = 0) -> Case:
     :func:`_healthy` (rejection sampling keeps the stated distributions)."""
     for attempt in range(64):
         c = _area(seed + attempt * 1_000_003, 118, 53, 69, 9, 42.0, id0=id0,
-                  pv_share=1.0, load_on_pv=True, qlim_scale=2.0)
+                  pv_share=1.0, load_on_pv=True, qlim_scale=2.0, root_at_slack=True)
         if _healthy(c):
             return c
     raise RuntimeError(f"no healthy 118-bus area found for seed {seed}")

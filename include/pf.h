@@ -1,0 +1,206 @@
+/*
+ * pf.h -- C ABI of the MI355X Newton-Raphson power-flow library (libpf.so).
+ *
+ * Plain C types only: ints, doubles, pointers and sizes.  Every entry point
+ * returns a pf_status; no entry point exits, asserts or throws.
+ *
+ * Reference interfaces each entry point replaces (joaovdd/Experience-of-Using-
+ * GPU-in-Power-Flow-Computation, paths relative to the repository root):
+ *
+ *   pf_options_parse / pf_options_load   loadFile()          CPU/leitura/config.cpp:5-166,
+ *                                                            GPU/sistInfo.h:62-242
+ *   pf_read_case                         lerArquivoEAlocarMemoria() CPU/leitura/arquivo.cpp:3-51
+ *                                        + readCDF/readCDFX  CPU/leitura/cdf.cpp:163-493
+ *   pf_solver_create                     calcYbusSp_Matpower() CPU/esparso/spCalcYbus.cpp:4-90,
+ *                                        initGPU/d_init*      GPU/sistema.cuh:12-469,
+ *                                        initLim             CPU/limInjReat.cpp:177-207
+ *   pf_solve                             nR()                CPU/newtonRaphson.cpp:173-314,
+ *                                        mNR()               GPU/newtonRaphson.cuh:171-623
+ *   pf_stage_ybus                        calcYbusSp_Matpower CPU/esparso/spCalcYbus.cpp:4-90
+ *   pf_stage_pq                          spCalcP/spCalcQ     CPU/esparso/potencia/spCalcPQliq.cpp:3-45,
+ *                                        calcP/Qeficiente_0based GPU/PQcalc.cuh:177-219
+ *   pf_stage_mismatch                    calcResLim          CPU/limInjReat.cpp:158-175,
+ *                                        calcResLimf         GPU/newtonRaphson.cuh:35-74
+ *   pf_stage_jacobian                    spCalcJ_eficiente   CPU/esparso/spCalcJac.cpp:402-568,
+ *                                        SpCalcJac           GPU/Jacobiano.cuh:382-415
+ *   pf_stage_lu_solve                    solver()/spSolveMKL CPU/solver.cpp:3-24,
+ *                                        CPU/esparso/spSolver.cpp:84-189; cusolverSpDcsrlsvqr
+ *                                        GPU/newtonRaphson.cuh:466-468
+ *   pf_stage_limits                      chkLimQ/undo_chkLimQ/geraVetPV/geraVetPQ
+ *                                        CPU/limInjReat.cpp:3-131, CPU/newtonRaphson.cpp:64-154
+ *   pf_batch_*                           (no counterpart: the reference solves one case per run)
+ *   pf_warmstart_*                       (no counterpart: learned initial state)
+ */
+#ifndef PF_H
+#define PF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pf_status {
+    PF_OK = 0,
+    PF_NOT_CONVERGED = 1,   /* no_max_iter reached (the reference only printed this) */
+    PF_SINGULAR = 2,        /* zero or non-finite pivot in the Jacobian factorisation */
+    PF_PARSE_ERROR = 3,     /* malformed CDF/CDFX or data.txt                         */
+    PF_DEVICE_ERROR = 4,    /* HIP runtime failure, or no GPU                          */
+    PF_INVALID_ARG = 5,
+    PF_IO_ERROR = 6         /* file cannot be opened                                  */
+} pf_status;
+
+/* metodo (CPU/estrutura/metodo.h:2-10, GPU/sistInfo.h:94-118) */
+typedef enum pf_metodo {
+    PF_ESPARSO = 0,         /* GPU Jacobian + GPU sparse LU                           */
+    PF_DENSO = 1,           /* GPU Jacobian + GPU dense LU with partial pivoting      */
+    PF_HIBRIDO_C = 2        /* GPU Jacobian + host sparse LU (the reference: PARDISO) */
+} pf_metodo;
+
+typedef enum pf_precision {
+    PF_FP64 = 0,            /* FP64 factor                                            */
+    PF_MIXED = 1            /* FP32 factor + FP64 iterative refinement                */
+} pf_precision;
+
+/* data.txt keys; the last three are this library's own additions. */
+typedef struct pf_options {
+    int metodo;             /* pf_metodo                                              */
+    int lim_inj_reat;       /* PV->PQ switching (default OFF, CPU/global.cpp:14)      */
+    int openmp;
+    double v_inicial, theta_inicial, tolerancia;
+    int no_max_iter;
+    char arq_entrada[1024];
+    int verbose_mode, laconic_mode;
+    int output_bench;       /* 0 ALL, 1 SCREEN, 2 FILE                                */
+    int output_ans, output_iter, streams;
+    int precision;          /* pf_precision                                           */
+    int refine_steps;       /* FP64 refinement steps after an FP32 solve (default 2)  */
+    int use_graph;          /* 1: one captured hipGraph per NR iteration, no per-iteration host sync */
+} pf_options;
+
+const char *pf_status_str(pf_status s);
+
+void pf_options_default(pf_options *o);
+/* cpu_layout = 1 prefixes "sistemas/" to arq_entrada, as the CPU build does
+ * (CPU/leitura/config.cpp:108); 0 keeps it as written (GPU/sistInfo.h:153). */
+pf_status pf_options_parse(const char *text, int cpu_layout, pf_options *out);
+pf_status pf_options_load(const char *path, int cpu_layout, pf_options *out);
+
+/* ---------------------------------------------------------------- network */
+typedef struct pf_network pf_network;
+
+typedef struct pf_network_info {
+    int nB, nL, nPV, nPQ;
+    int slack;              /* 0-based index of the Vtheta bus                        */
+    int n_duplicates;       /* parallel branches merged by the reader                 */
+    double baseMVA;
+} pf_network_info;
+
+enum pf_bus_field {
+    PF_BUS_V = 0, PF_BUS_THETA, PF_BUS_PLIQ, PF_BUS_QLIQ, PF_BUS_PLOAD, PF_BUS_QLOAD,
+    PF_BUS_PG, PF_BUS_QG, PF_BUS_VBASE, PF_BUS_GSH, PF_BUS_BSH, PF_BUS_NFIELDS
+};
+
+/* CDF (.cdf) or CDFX (.cdfx) by extension; v0/theta0 = flat-start values.    */
+pf_status pf_read_case(const char *path, double v0, double theta0, pf_network **out);
+void pf_network_free(pf_network *net);
+pf_status pf_network_get_info(const pf_network *net, pf_network_info *info);
+pf_status pf_network_get_bus(const pf_network *net, int field, double *out /* nB */);
+/* any pointer may be NULL */
+pf_status pf_network_get_lists(const pf_network *net, int *pv, int *pq, double *qmin,
+                               double *qmax, double *vset);
+pf_status pf_network_get_branches(const pf_network *net, int *from, int *to, double *r,
+                                  double *x, double *bch, double *tap_re, double *tap_im);
+
+/* ----------------------------------------------------------------- solver */
+typedef struct pf_solver pf_solver;
+
+typedef struct pf_result {
+    int iterations;         /* updates applied (the reference's iterativo->iteracao)  */
+    int converged;
+    int singular;
+    double max_mismatch;    /* max |g| at the last mismatch evaluation (pu)           */
+    int n_switch_events;    /* PV<->PQ flips over the run                             */
+    double solve_ms;        /* wall time of the NR loop (host clock)                   */
+} pf_result;
+
+/* Builds Ybus on the GPU, the ordering and symbolic factorisation (once),
+ * uploads everything to `device` and sets the flat-start state. */
+pf_status pf_solver_create(const pf_network *net, const pf_options *opt, int device,
+                           pf_solver **out);
+void pf_solver_destroy(pf_solver *s);
+/* NR from the current state (flat start after create / reset). */
+pf_status pf_solve(pf_solver *s, pf_result *res);
+pf_status pf_solver_reset(pf_solver *s);                 /* back to flat start */
+pf_status pf_solver_set_state(pf_solver *s, const double *V, const double *theta);
+pf_status pf_solver_set_injections(pf_solver *s, const double *Pliq, const double *Qliq,
+                                   const double *Qload);
+pf_status pf_solver_get_state(pf_solver *s, double *V, double *theta,
+                              unsigned char *limQ /* nPV, may be NULL */);
+/* sizes of the factorisation: n = nB-1 block unknowns, nnz of strict L in
+ * 2x2 blocks, elimination-tree levels, levels run by one workgroup */
+pf_status pf_solver_lu_info(pf_solver *s, int *n, int *nnzL, int *levels, int *tail_levels);
+
+/* ---------------------------------------------------- stage entry points */
+/* Host buffers in and out; each stage runs its own kernels on the GPU.      */
+/* Ybus CSR (0-based, sorted columns), y interleaved re/im: 2*nnz doubles.   */
+pf_status pf_stage_ybus(pf_solver *s, int *rowptr, int *colind, double *y, int *nnz);
+pf_status pf_stage_pq(pf_solver *s, const double *V, const double *theta, double *P,
+                      double *Q);
+/* limQ [nPV] and QliqLim [nB] give the PV->PQ state; g gets nB-1+nPQlim
+ * entries in the reference's order; returns max|g|.                         */
+pf_status pf_stage_mismatch(pf_solver *s, const double *V, const double *theta,
+                            const unsigned char *limQ, const double *QliqLim, double *g,
+                            double *max_abs);
+/* Jacobian in the reference's reduced CSR layout (rows: P of the non-slack
+ * buses, Q of barrasPQlim; columns: theta of the non-slack buses, |V| of
+ * barrasPQlim).  rowptr has nJ+1 entries, col/val hold nnz (cap entries).   */
+pf_status pf_stage_jacobian(pf_solver *s, const double *V, const double *theta,
+                            const unsigned char *limQ, int *nJ, int *rowptr, int *col,
+                            double *val, int cap, int *nnz);
+/* factor the Jacobian at (V, theta, limQ) and solve J x = rhs (reduced layout) */
+pf_status pf_stage_lu_solve(pf_solver *s, const double *V, const double *theta,
+                            const unsigned char *limQ, const double *rhs, double *x);
+/* one limInjReat decision pass (mode 0: chkLimQ, 1: undo_chkLimQ) at state
+ * (V, Q); limQ/QliqLim in/out; returns the number of flips.                 */
+pf_status pf_stage_limits(pf_solver *s, int mode, const double *V, const double *Q,
+                          unsigned char *limQ, double *QliqLim, int *flips);
+
+/* text of the last HIP runtime failure seen by this thread ("" if none)      */
+const char *pf_last_error(void);
+
+/* ------------------------------------------------------ host-only probes */
+/* The ordering / symbolic factor program without a GPU (CPU tests, and the
+ * host factorisation behind metodo hibridoC).  Slots: [0,n) diagonal blocks,
+ * [n, n+nnzL) strict-lower blocks, [n+nnzL, n+2nnzL) strict-upper blocks.   */
+typedef struct pf_symbolic_probe pf_symbolic_probe;
+pf_status pf_probe_create(const pf_network *net, pf_symbolic_probe **out, int *n, int *nnzL,
+                          int *nslot, int *levels);
+void pf_probe_free(pf_symbolic_probe *p);
+/* perm[j] = block (bus order without the slack) eliminated j-th; slot_e[s] =
+ * Ybus entry whose Jacobian block lands in slot s (-1: fill-in)             */
+pf_status pf_probe_get(const pf_symbolic_probe *p, int *perm, int *slot_e, int *rowptr,
+                       int *colind, int *ynnz);
+/* blocks: nslot 2x2 blocks (row-major), x: 2n right-hand side in node order */
+pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x);
+
+/* ---------------------------------------------------------------- batches */
+/* Many scenarios of one or more topologies solved in one persistent launch:
+ * one workgroup owns one scenario for its whole NR solve.                   */
+typedef struct pf_batch pf_batch;
+
+pf_status pf_batch_create(const pf_network *const *nets, int n_nets, const pf_options *opt,
+                          int device, pf_batch **out);
+void pf_batch_destroy(pf_batch *b);
+/* scenario s uses network net_of[s]; Pliq/Qliq/Qload rows of that network's
+ * nB, packed back to back in scenario order.  V0/theta0 (may be NULL = flat
+ * start) are packed the same way. */
+pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of,
+                                 const double *Pliq, const double *Qliq, const double *Qload,
+                                 const double *V0, const double *theta0);
+pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms);
+pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations,
+                       int *converged, double *max_mismatch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -53,6 +53,7 @@ def lib():
                                    ip, ip, dp, C.c_int]
         L.orc_sparse_solve.argtypes = [C.c_int, ip, ip, dp, dp, dp]
         L.orc_min_degree.argtypes = [C.c_int, ip, ip, ip]
+        L.orc_limits.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.orc_nr.argtypes = [P, C.POINTER(OrcOpts), P, P, P, P, P, P, C.POINTER(OrcResult)]
         L.orc_nr_batch.argtypes = [P, C.POINTER(OrcOpts), C.c_int, P, P, P, P, P, P, P, C.c_int]
         _lib = L
@@ -115,6 +116,15 @@ class Case:
         return dict(V=V, theta=th, limQ=limQ[:self.nPV].astype(bool), iterations=r.iterations,
                     converged=bool(r.converged), singular=bool(r.singular),
                     max_mismatch=r.max_mismatch, n_switch_events=r.n_switch_events)
+
+    def limits(self, mode, Q, V, limQ, QliqLim, Qload=None, Qliq=None):
+        lq = np.ascontiguousarray(limQ, np.uint8).copy()
+        ql = np.ascontiguousarray(QliqLim, np.float64).copy()
+        Qload = self.Qload if Qload is None else Qload
+        Qliq = self.Qliq if Qliq is None else Qliq
+        a = [np.ascontiguousarray(x, np.float64) for x in (Qload, Qliq, Q, V)]
+        f = lib().orc_limits(self.h, mode, *[_ptr(x) for x in a], _ptr(lq), _ptr(ql))
+        return lq.astype(bool), ql, f
 
     def solve_batch(self, Pliq, Qliq, Qload, tol=1e-8, max_iter=100, lim=True,
                     reuse_symbolic=False, nthreads=1):

@@ -422,8 +422,9 @@ void orc_ybus(const orc_case *c, int *rowptr, int *colind, double *y) {
         int f = c->from[l], t = c->to[l];
         double yr, yi;
         cdiv(1.0, 0.0, c->r[l], c->x[l], &yr, &yi);                  /* 1/z */
-        double ta = hypot(c->tap_re[l], c->tap_im[l]);
-        double s = 1.0 / (ta * ta);
+        /* 1/|t|^2 as 1/(re^2 + im^2): the reference squares cuCabs(t); the
+         * product form has no library call, so it rounds the same everywhere */
+        double s = 1.0 / (c->tap_re[l] * c->tap_re[l] + c->tap_im[l] * c->tap_im[l]);
         double ffr = yr * s, ffi = (yi + c->bch[l] / 2.) * s;        /* (y + jb/2)/|t|^2 */
         double a_r, a_i, ftr, fti, tfr, tfi;
         cdiv(-1.0, 0.0, c->tap_re[l], -c->tap_im[l], &a_r, &a_i);   /* -1/conj(t) */
@@ -896,6 +897,16 @@ static double maxabs(const double *v, int n) {
     double m = 0;
     for (int i = 0; i < n; i++) if (fabs(v[i]) > m) m = fabs(v[i]);
     return m;
+}
+
+/* exported single decision passes (stage tests): mode 0 chkLimQ, 1 undo_chkLimQ */
+int orc_limits(const orc_case *c, int mode, const double *Qload, const double *Qliq,
+               const double *Q, const double *V, unsigned char *limQ, double *QliqLim) {
+    limstate s;
+    s.limQ = limQ;
+    s.QliqLim = QliqLim;
+    if (mode == 0) return chk_lim_q(c, Qload, Qliq, Q, V, &s);
+    return undo_chk_lim_q(c, Qload, Qliq, V, &s);
 }
 
 /* ------------------------------------------------------------------------- */

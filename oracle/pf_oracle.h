@@ -83,6 +83,11 @@ int  orc_sparse_solve(int n, const int *rowptr, const int *colind, const double 
 /* the ordering alone (for tests): perm[k] = original index eliminated k-th   */
 void orc_min_degree(int n, const int *rowptr, const int *colind, int *perm);
 
+/* --- one limInjReat decision pass (CPU/limInjReat.cpp:38-131): mode 0 chkLimQ,
+ *     1 undo_chkLimQ; limQ [nPV] and QliqLim [nB] in/out; returns flips     */
+int  orc_limits(const orc_case *c, int mode, const double *Qload, const double *Qliq,
+                const double *Q, const double *V, unsigned char *limQ, double *QliqLim);
+
 /* --- Newton-Raphson driver (CPU/newtonRaphson.cpp:173-314) ------------------- */
 /* V/theta in: initial state (NULL = the case's), out: final state.           */
 /* limQ_out [nPV] (may be NULL).  Pliq/Qliq/Qload override the case (NULL=case) */

@@ -1,0 +1,183 @@
+"""Host-side checks of libpf that need no GPU: the C ABI exports, the reader
+(bit-exact against the oracle's), the data.txt parser and its quirks, the
+ordering / symbolic factor program, and the case generator."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, IEEE, PKG_DIR, ROOT
+
+HEADER = os.path.join(ROOT, "include", "pf.h")
+
+
+def test_exports_every_declared_symbol(pf):
+    text = open(HEADER).read()
+    names = set(re.findall(r"^\s*(?:const\s+char\s*\*|pf_status|void)\s*\*?\s*(pf_\w+)\s*\(", text, re.M))
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(PKG_DIR, "libpf.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(pf_\w+)", out))
+    missing = names - exported
+    assert not missing, missing
+    lib = pf.lib()
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def _paths(cases):
+    return [os.path.join(IEEE, f"ieee{n}.cdf") for n in (14, 30, 57, 118)] + \
+           [os.path.join(GOLDEN, "c14.cdf"), os.path.join(GOLDEN, "c118.cdf"), cases.path("c944")]
+
+
+def test_reader_parity_with_oracle(pf, oracle, cases):
+    for p in _paths(cases):
+        a = pf.Network(p)
+        b = oracle.Case(p)
+        assert (a.nB, a.nL, a.nPV, a.nPQ, a.slack, a.n_duplicates) == \
+               (b.nB, b.nL, b.nPV, b.nPQ, b.slack, b.n_duplicates)
+        for f in ["V", "theta", "Pliq", "Qliq", "Pload", "Qload", "Pg", "Qg", "Vbase", "gsh", "bsh",
+                  "Qmin", "Qmax", "Vset", "r", "x", "bch", "tap_re", "tap_im"]:
+            assert np.array_equal(getattr(a, f), getattr(b, f)), (p, f)
+        for f in ["pv", "pq", "frm", "to"]:
+            assert np.array_equal(getattr(a, f), getattr(b, f)), (p, f)
+
+
+def test_cdfx_layout(pf, oracle, synth, tmp_path):
+    c = synth.case14()
+    p1, p2 = str(tmp_path / "a.cdf"), str(tmp_path / "a.cdfx")
+    synth.write_cdf(c, p1, wide=False)
+    synth.write_cdf(c, p2, wide=True)
+    a, b = pf.Network(p1), pf.Network(p2)
+    for f in ["V", "Pliq", "Qliq", "Qmax", "r", "x", "bch", "tap_re"]:
+        assert np.array_equal(getattr(a, f), getattr(b, f))
+    assert np.array_equal(oracle.Case(p2).x, b.x)
+
+
+def test_reader_errors(pf, tmp_path):
+    src = open(os.path.join(GOLDEN, "c14.cdf")).read().split("\n")
+    def write(name, lines):
+        p = str(tmp_path / name)
+        open(p, "w").write("\n".join(lines))
+        return p
+    two_slack = write("a.cdf", src[:3] + [src[3][:24] + " 3" + src[3][26:]] + src[4:])
+    bad_type = write("b.cdf", src[:3] + [src[3][:24] + " 7" + src[3][26:]] + src[4:])
+    j = next(i for i, l in enumerate(src) if l.startswith("BRANCH")) + 1
+    unknown = write("c.cdf", src[:j] + ["9999" + src[j][4:]] + src[j + 1:])
+    for p in (two_slack, bad_type, unknown):
+        with pytest.raises(pf.PfError) as e:
+            pf.Network(p)
+        assert e.value.status == pf.PF_PARSE_ERROR
+    with pytest.raises(pf.PfError) as e:
+        pf.Network(str(tmp_path / "missing.cdf"))
+    assert e.value.status == pf.PF_IO_ERROR
+    with pytest.raises(pf.PfError) as e:
+        pf.Network(str(tmp_path / "case.pwf"))
+    assert e.value.status == pf.PF_INVALID_ARG
+
+
+DATA_TXT = """metodo         = {metodo};        // esparso, denso or hibridoC
+openmp         = OFF;
+v_inicial      = 1.02;            // flat start
+theta_inicial  = 0.0;
+tolerancia     = 1e-8;
+no_max_iter    = 37;
+arq_entrada    = ieee118.cdf;     // input file name
+laconic_mode   = ON;
+output_bench   = SCREEN;
+output_ans     = OFF;
+{extra}"""
+
+
+def test_options_parse(pf):
+    o = pf.options(DATA_TXT.format(metodo="hibridoC", extra="lim_inj_reat = ON;\n"), cpu_layout=0)
+    assert o.metodo == pf.HIBRIDO_C and o.lim_inj_reat == 1 and o.openmp == 0
+    assert o.v_inicial == 1.02 and o.tolerancia == 1e-8 and o.no_max_iter == 37
+    assert o.arq_entrada == b"ieee118.cdf" and o.laconic_mode == 1 and o.output_ans == 0
+    assert o.output_bench == 1
+    o = pf.options(DATA_TXT.format(metodo="esparso", extra=""), cpu_layout=1)
+    # CPU build prefixes the systems folder (CPU/leitura/config.cpp:108)
+    assert o.arq_entrada == b"sistemas/ieee118.cdf" and o.metodo == pf.ESPARSO
+    # a key missing from the file reads the value after the file's first '='
+    # ("esparso"), which is neither ON nor OFF: lim_inj_reat keeps its default
+    assert o.lim_inj_reat == 0
+    o = pf.options(DATA_TXT.format(metodo="denso", extra="precisao = MISTA;\ngrafo = ON;\n"))
+    assert o.metodo == pf.DENSO and o.precision == pf.MIXED and o.use_graph == 1
+
+
+def test_options_errors(pf, tmp_path):
+    with pytest.raises(pf.PfError) as e:
+        pf.options(DATA_TXT.format(metodo="hibridoA", extra=""))
+    assert e.value.status == pf.PF_INVALID_ARG
+    with pytest.raises(pf.PfError) as e:
+        pf.options(DATA_TXT.format(metodo="esparso", extra="").replace("= 37", "= abc"))
+    assert e.value.status == pf.PF_PARSE_ERROR
+    o = pf.Options()
+    assert pf.lib().pf_options_load(str(tmp_path / "none.txt").encode(), 0, pf.C.byref(o)) == pf.PF_IO_ERROR
+
+
+def _dense_from_slots(net, pr, blocks):
+    n, sl = pr.n, net.slack
+    iperm = np.empty(n, int)
+    iperm[pr.perm] = np.arange(n)
+    coorow = np.repeat(np.arange(net.nB), np.diff(pr.rowptr))
+    A = np.zeros((2 * n, 2 * n))
+    for s in range(pr.nslot):
+        e = pr.slot_e[s]
+        if e < 0:
+            continue
+        k, m = coorow[e], pr.colind[e]
+        a, c = iperm[k - (k > sl)], iperm[m - (m > sl)]
+        A[2 * a:2 * a + 2, 2 * c:2 * c + 2] = blocks[s].reshape(2, 2)
+    return A
+
+
+@pytest.mark.parametrize("which", ["ieee14", "ieee118", "c118", "c944"])
+def test_symbolic_program_solves(pf, cases, which):
+    path = os.path.join(IEEE, which + ".cdf") if which.startswith("ieee") else cases.path(which)
+    net = pf.Network(path)
+    pr = pf.SymbolicProbe(net)
+    assert sorted(pr.perm) == list(range(pr.n))
+    # every Ybus entry off the slack lands in exactly one slot
+    used = pr.slot_e[pr.slot_e >= 0]
+    coorow = np.repeat(np.arange(net.nB), np.diff(pr.rowptr))
+    off_slack = np.sum((coorow != net.slack) & (pr.colind != net.slack))
+    assert len(used) == len(set(used)) == off_slack
+    rng = np.random.default_rng(7)
+    blocks = np.zeros((pr.nslot, 4))
+    for s in range(pr.nslot):
+        if pr.slot_e[s] >= 0:
+            blocks[s] = rng.standard_normal(4) + (np.array([8, 0, 0, 8]) if s < pr.n else 0)
+    b = rng.standard_normal(2 * pr.n)
+    x = pr.factor_solve(blocks, b)
+    xr = np.linalg.solve(_dense_from_slots(net, pr, blocks), b)
+    assert np.abs(x - xr).max() <= 1e-12 * np.abs(xr).max()
+
+
+def test_splitmix64_known_answers(synth):
+    # published outputs of SplitMix64 (Steele, Lea & Flood 2014) for seeds 0 and 1234567
+    r = synth.SplitMix64(0)
+    assert [r.next_u64() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    r = synth.SplitMix64(1234567)
+    assert [r.next_u64() for _ in range(3)] == [6457827717110365317, 3203168211198807973, 9817491932198370423]
+
+
+def test_generator_workloads(pf, synth, cases, tmp_path):
+    c = synth.case118()
+    assert len(c.buses) == 118 and len(c.branches) == 186
+    assert sum(b.type == 2 for b in c.buses) == 53 and sum(b.type == 0 for b in c.buses) == 64
+    assert sum(br.tap != 0 for br in c.branches) == 9
+    assert all(br.b == 0 for br in c.branches if br.tap != 0)
+    assert abs(sum(b.pload for b in c.buses) / 100 - 42.0) < 0.05
+    # the written file is a pure function of the seed
+    p = str(tmp_path / "again.cdf")
+    synth.write_cdf(synth.case118(), p)
+    assert open(p).read() == open(os.path.join(GOLDEN, "c118.cdf")).read()
+    n = pf.Network(cases.path("c944"))
+    assert (n.nB, n.nPV, n.nPQ, n.nL) == (944, 8 * 53 + 7, 8 * 64, 8 * 186 + 16)
+    c14 = synth.case14()
+    assert (sum(b.type == 2 for b in c14.buses), sum(b.type == 0 for b in c14.buses), len(c14.branches)) == (4, 9, 20)
+    f = synth.load_factors(118, 3000000)
+    assert f.min() >= 0.7 and f.max() <= 1.3 and np.array_equal(f, synth.load_factors(118, 3000000))

@@ -1,0 +1,868 @@
+// Batched NR: many scenarios (perturbed injections, optional warm start) of
+// one or more topologies solved in one persistent launch per topology.
+//
+// One workgroup owns one scenario for its whole NR solve and pulls the next
+// scenario from a per-topology work counter when done, so uneven scenario
+// sizes and iteration counts balance themselves.  Small topologies keep the
+// whole working set (state vectors, mismatch, step, 2x2-block LU) in LDS;
+// larger ones use a per-workgroup slice of a global workspace (L2 resident).
+// The topology (Ybus, symbolic factor program) is shared read-only.
+//
+// Per scenario the pass is the same as the single-network solver
+// (CPU/newtonRaphson.cpp:180-305): P,Q -> limits trigger -> mismatch ->
+// convergence / switch at convergence -> Jacobian -> FP32 block LU ->
+// solves -> FP64 refinement -> update.  A start that diverges (non-finite or
+// huge mismatch, singular pivot, or no convergence) is retried from the flat
+// start inside the same workgroup.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels_decl.hpp"
+#include "kernels_impl.hpp"
+#include "pf_internal.hpp"
+
+using namespace pf;
+
+namespace pf {
+
+struct BatchTopo {
+    DevNet N;
+    DevSym Y;
+    const double *V0flat, *th0flat;     // flat start of this topology
+    int nslot;
+};
+
+struct BatchJob {
+    int nscen;                          // scenarios of this topology
+    const int *scen;                    // global scenario index
+    const long long *off;               // bus offset of scenario s in the packed arrays
+    const double *Pliq, *Qliq, *Qload;  // packed [sum nB]
+    const double *V0, *th0;             // packed warm start, may be null
+    double *Vout, *thout;               // packed results
+    int *iters, *conv;
+    double *maxmis;
+    int *counter;                       // work queue head
+    char *gws;                          // global workspace (per workgroup slice)
+    long long ws_bytes;                 // bytes per workgroup slice
+    int max_iter, lim, refine;
+    double tol, diverge;
+};
+
+constexpr int GL = 16;   // lanes per node in the in-workgroup LU and solves
+
+// per-scenario working set, in LDS or in a global slice
+template <typename T>
+struct Work {
+    double *V, *th, *P, *Q, *ql, *g, *x, *r;
+    double2 *E, *I;
+    unsigned char *isq, *limQ;
+    T *blk;
+};
+
+template <typename T>
+__host__ __device__ inline long long work_bytes(int nB, int n, int nPV, int nslot) {
+    long long b = 0;
+    b += 8LL * nB * 5;                 // V th P Q ql
+    b += 8LL * 2 * n * 3;              // g x r
+    b += 16LL * nB * 2;                // E I
+    b = (b + 15) & ~15LL;
+    b += (long long)sizeof(T) * 4 * nslot;
+    b += nB + nPV + 16;
+    return (b + 255) & ~255LL;
+}
+
+template <typename T>
+__device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nslot) {
+    Work<T> w;
+    char *p = base;
+    auto take = [&](size_t bytes) { char *q = p; p += bytes; return q; };
+    w.V = (double *)take(8 * (size_t)nB);
+    w.th = (double *)take(8 * (size_t)nB);
+    w.P = (double *)take(8 * (size_t)nB);
+    w.Q = (double *)take(8 * (size_t)nB);
+    w.ql = (double *)take(8 * (size_t)nB);
+    w.g = (double *)take(16 * (size_t)n);
+    w.x = (double *)take(16 * (size_t)n);
+    w.r = (double *)take(16 * (size_t)n);
+    w.E = (double2 *)take(16 * (size_t)nB);
+    w.I = (double2 *)take(16 * (size_t)nB);
+    p = base + ((p - base + 15) & ~(size_t)15);
+    w.blk = (T *)take(sizeof(T) * 4 * (size_t)nslot);
+    w.isq = (unsigned char *)take(nB);
+    w.limQ = (unsigned char *)take(nPV);
+    return w;
+}
+
+template <typename T>
+__device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *bad) {
+    const int d0 = Y.fd_ptr[j], nd = Y.fd_ptr[j + 1] - d0;
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL;
+    B4<T> dinv{0, 0, 0, 0};
+    for (int base = 0; base < nd; base += GL) {
+        const int t = base + lane;
+        B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+        int p = -1;
+        if (t < nd) {
+            const int d = d0 + t;
+            if (t == 0) {
+                a = load_b4(blk, j);
+                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++)
+                    mms(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
+            } else {
+                p = Y.colptr[j] + t - 1;
+                a = load_b4(blk, L0 + p);
+                u = load_b4(blk, U0 + p);
+                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++) {
+                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
+                    mms(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
+                    mms(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
+                }
+            }
+        }
+        if (base == 0) {
+            B4<T> di{0, 0, 0, 0};
+            if (lane == 0) {
+                T det = a.a * a.d - a.b * a.c;
+                if (!(det != (T)0) || !isfinite((double)det)) {
+                    *bad = 1;
+                    det = (T)1;
+                }
+                di = B4<T>{a.d / det, -a.b / det, -a.c / det, a.a / det};
+                store_b4(blk, j, di);
+            }
+            dinv.a = __shfl(di.a, 0, GL);
+            dinv.b = __shfl(di.b, 0, GL);
+            dinv.c = __shfl(di.c, 0, GL);
+            dinv.d = __shfl(di.d, 0, GL);
+        }
+        if (t < nd && t > 0) {
+            store_b4(blk, L0 + p, mul(a, dinv));
+            store_b4(blk, U0 + p, u);
+        }
+    }
+}
+
+template <typename T>
+__device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
+    const int grp = (int)threadIdx.x / GL, lane = (int)threadIdx.x % GL, ng = (int)blockDim.x / GL;
+    for (int l = 0; l < Y.nlev; l++) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        for (int i = grp; i < c; i += ng) {
+            const int j = Y.lvl_nodes[s + i];
+            double a0 = 0, a1 = 0;
+            for (int q = Y.rs_ptr[j] + lane; q < Y.rs_ptr[j + 1]; q += GL) {
+                B4<T> L = load_b4(blk, Y.n + Y.rs_p[q]);
+                const int k = Y.rs_k[q];
+                a0 += (double)L.a * x[2 * k] + (double)L.b * x[2 * k + 1];
+                a1 += (double)L.c * x[2 * k] + (double)L.d * x[2 * k + 1];
+            }
+#pragma unroll
+            for (int off = GL / 2; off > 0; off >>= 1) {
+                a0 += __shfl_xor(a0, off, GL);
+                a1 += __shfl_xor(a1, off, GL);
+            }
+            if (lane == 0) {
+                x[2 * j] -= a0;
+                x[2 * j + 1] -= a1;
+            }
+        }
+        __syncthreads();
+    }
+    const int U0 = Y.n + Y.nnzL;
+    for (int l = Y.nlev - 1; l >= 0; l--) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        for (int i = grp; i < c; i += ng) {
+            const int j = Y.lvl_nodes[s + i];
+            double a0 = 0, a1 = 0;
+            for (int p = Y.colptr[j] + lane; p < Y.colptr[j + 1]; p += GL) {
+                B4<T> U = load_b4(blk, U0 + p);
+                const int r = Y.rowind[p];
+                a0 += (double)U.a * x[2 * r] + (double)U.b * x[2 * r + 1];
+                a1 += (double)U.c * x[2 * r] + (double)U.d * x[2 * r + 1];
+            }
+#pragma unroll
+            for (int off = GL / 2; off > 0; off >>= 1) {
+                a0 += __shfl_xor(a0, off, GL);
+                a1 += __shfl_xor(a1, off, GL);
+            }
+            if (lane == 0) {
+                const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
+                B4<T> D = load_b4(blk, j);
+                x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
+                x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// I = Y E and S = E conj(I): 4 lanes per row
+template <typename T>
+__device__ void wg_pq(const DevNet &N, Work<T> &w) {
+    const int rows_per = (int)blockDim.x / 4;
+    const int lane = (int)threadIdx.x & 3;
+    for (int r0 = 0; r0 < N.nB; r0 += rows_per) {
+        const int row = r0 + (int)threadIdx.x / 4;
+        double ar = 0, ai = 0;
+        if (row < N.nB) {
+            for (int e = N.rowptr[row] + lane; e < N.rowptr[row + 1]; e += 4) {
+                const double2 Em = w.E[N.colind[e]];
+                const double2 y = N.y[e];
+                ar += y.x * Em.x - y.y * Em.y;
+                ai += y.x * Em.y + y.y * Em.x;
+            }
+        }
+        ar += __shfl_xor(ar, 1, 4);
+        ai += __shfl_xor(ai, 1, 4);
+        ar += __shfl_xor(ar, 2, 4);
+        ai += __shfl_xor(ai, 2, 4);
+        if (row < N.nB && lane == 0) {
+            const double2 Ek = w.E[row];
+            w.I[row] = make_double2(ar, ai);
+            w.P[row] = Ek.x * ar + Ek.y * ai;
+            w.Q[row] = Ek.y * ar - Ek.x * ai;
+        }
+    }
+}
+
+// r = g - J x (FP64, complex voltages)
+template <typename T>
+__device__ void wg_residual(const DevNet &N, Work<T> &w) {
+    const int rows_per = (int)blockDim.x / 4;
+    const int lane = (int)threadIdx.x & 3;
+    for (int r0 = 0; r0 < N.nB; r0 += rows_per) {
+        const int row = r0 + (int)threadIdx.x / 4;
+        const bool active = row < N.nB && row != N.slack;
+        double ar = 0, ai = 0;
+        if (active) {
+            for (int e = N.rowptr[row] + lane; e < N.rowptr[row + 1]; e += 4) {
+                const int m = N.colind[e];
+                if (m == N.slack) continue;
+                const int jm = N.node_of_bus[m];
+                const double dth = w.x[2 * jm];
+                const double dv = w.isq[m] ? w.x[2 * jm + 1] : 0.0;
+                const double2 Em = w.E[m];
+                const double f = dv / w.V[m];
+                const double dr = Em.x * f - Em.y * dth, di = Em.y * f + Em.x * dth;
+                const double2 y = N.y[e];
+                ar += y.x * dr - y.y * di;
+                ai += y.x * di + y.y * dr;
+            }
+        }
+        ar += __shfl_xor(ar, 1, 4);
+        ai += __shfl_xor(ai, 1, 4);
+        ar += __shfl_xor(ar, 2, 4);
+        ai += __shfl_xor(ai, 2, 4);
+        if (active && lane == 0) {
+            const int j = N.node_of_bus[row];
+            const double dth = w.x[2 * j];
+            const bool q = w.isq[row];
+            const double dv = q ? w.x[2 * j + 1] : 0.0;
+            const double2 Ek = w.E[row], Ik = w.I[row];
+            const double f = dv / w.V[row];
+            const double dr = Ek.x * f - Ek.y * dth, di = Ek.y * f + Ek.x * dth;
+            const double sr = dr * Ik.x + di * Ik.y + Ek.x * ar + Ek.y * ai;
+            const double si = di * Ik.x - dr * Ik.y + Ek.y * ar - Ek.x * ai;
+            w.r[2 * j] = w.g[2 * j] - sr;
+            w.r[2 * j + 1] = q ? w.g[2 * j + 1] - si : -w.x[2 * j + 1];
+        }
+    }
+}
+
+// limits pass: mode 0 chkLimQ (or undo when !chk), flips accumulated in *flips
+template <typename T>
+__device__ void wg_limits(const DevNet &N, Work<T> &w, const double *Qliq, const double *Qload,
+                          bool chk, int *flips) {
+    int f = 0;
+    for (int i = threadIdx.x; i < N.nPV; i += blockDim.x) {
+        const int b = N.pv_bus[i];
+        const double qload = Qload[b], qmin = N.qmin[i], qmax = N.qmax[i];
+        unsigned char lq = w.limQ[i];
+        double qll = w.ql[b];
+        if (!lq) {
+            if (chk) {
+                const double qg = w.Q[b] + qload;
+                if (qg < qmin - kFolga) { qll = qmin - qload; lq = 1; f++; }
+                else if (qg > qmax + kFolga) { qll = qmax - qload; lq = 1; f++; }
+            }
+        } else {
+            if (qll == qmax - qload) {
+                if (w.V[b] > N.vset[i] + kFolga) { lq = 0; qll = Qliq[b]; f++; }
+            } else {
+                if (w.V[b] < N.vset[i] - kFolga) { lq = 0; qll = Qliq[b]; f++; }
+            }
+        }
+        w.limQ[i] = lq;
+        w.ql[b] = qll;
+        w.isq[b] = lq;
+    }
+    if (f) atomicAdd(flips, f);
+}
+
+template <typename T>
+__device__ void wg_set_E(const DevNet &N, Work<T> &w) {
+    for (int k = threadIdx.x; k < N.nB; k += blockDim.x) {
+        double s, c;
+        sincos(w.th[k], &s, &c);
+        w.E[k] = make_double2(w.V[k] * c, w.V[k] * s);
+    }
+}
+
+// One NR solve of one scenario by the whole workgroup.  Returns (via shared
+// result) iterations, converged flag and final max mismatch.
+template <typename T>
+__device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const double *Pliq,
+                      const double *Qliq, const double *Qload, const double *V0, const double *th0,
+                      int *s_it, int *s_conv, double *s_maxg) {
+    const DevNet &N = tp.N;
+    const DevSym &Y = tp.Y;
+    __shared__ unsigned long long s_maxG_bits, s_maxDx_bits;
+    __shared__ int s_bad, s_flips, s_sing;
+    const int tid = threadIdx.x;
+    // a warm start sets |V| of the PQ buses and the angles of the non-slack
+    // buses; PV buses start at their setpoint and the slack at its data
+    for (int k = tid; k < N.nB; k += blockDim.x) {
+        const bool pq = N.pv_of_bus[k] < 0 && k != N.slack;
+        w.V[k] = pq ? V0[k] : tp.V0flat[k];
+        w.th[k] = (k == N.slack) ? tp.th0flat[k] : th0[k];
+        w.ql[k] = Qliq[k];
+        w.isq[k] = pq ? 1 : 0;                                   // initLim: PQ buses only
+    }
+    for (int i = tid; i < N.nPV; i += blockDim.x) w.limQ[i] = 0;
+    if (tid == 0) {
+        s_maxDx_bits = 0;
+        s_flips = 0;
+        s_sing = 0;
+    }
+    __syncthreads();
+    wg_set_E(N, w);
+    __syncthreads();
+    int it = 0, conv = 0;
+    double maxG = 0;
+    while (it < J.max_iter) {
+        wg_pq(N, w);
+        if (tid == 0) { s_maxG_bits = 0; s_bad = 0; }
+        __syncthreads();
+        if (J.lim && it >= 1) {
+            const bool chk = u64_as_double(s_maxDx_bits) < 0.1;
+            int any = 0;
+            if (!chk)
+                for (int i = tid; i < N.nPV; i += blockDim.x) any |= w.limQ[i];
+            any = __syncthreads_or(any);
+            if (chk || any) wg_limits(N, w, Qliq, Qload, chk, &s_flips);
+            __syncthreads();
+        }
+        // mismatch
+        {
+            double m = 0.0;
+            int bad = 0;
+            for (int k = tid; k < N.nB; k += blockDim.x) {
+                if (k == N.slack) continue;
+                const int j = N.node_of_bus[k];
+                const double gp = Pliq[k] - w.P[k];
+                const double gq = w.isq[k] ? w.ql[k] - w.Q[k] : 0.0;
+                w.g[2 * j] = gp;
+                w.g[2 * j + 1] = gq;
+                m = dmax_nan(m, dmax_nan(fabs(gp), fabs(gq)));
+                const int i = N.pv_of_bus[k];
+                if (i >= 0) {
+                    const double qg = w.Q[k] + Qload[k];
+                    bad |= (qg < N.qmin[i] - kFolga) || (qg > N.qmax[i] + kFolga);
+                }
+            }
+            m = wave_max(m);
+            bad = __any(bad);
+            if ((tid & 63) == 0) {
+                atomicMax(&s_maxG_bits, double_as_u64(m));
+                if (bad) atomicOr(&s_bad, 1);
+            }
+        }
+        __syncthreads();
+        maxG = u64_as_double(s_maxG_bits);
+        const int simple_bad = s_bad;
+        __syncthreads();
+        if (maxG <= J.tol) {
+            if (J.lim && simple_bad) {                    // switch taken at convergence
+                wg_limits(N, w, Qliq, Qload, true, &s_flips);
+                if (tid == 0) s_maxDx_bits = 0;
+                __syncthreads();
+                it++;
+                continue;
+            }
+            conv = 1;
+            break;
+        }
+        if (!(maxG < J.diverge)) break;                   // NaN or diverging: give up this start
+        // Jacobian blocks
+        for (int s = tid; s < tp.nslot; s += blockDim.x) {
+            const int e = Y.slot_e[s];
+            B4<T> out{0, 0, 0, 0};
+            if (e >= 0) {
+                const int k = N.coorow[e], m = N.colind[e];
+                const double2 y = N.y[e];
+                double H, Nn, M, L;
+                if (k != m) {
+                    const double2 Em = w.E[m], Ek = w.E[k];
+                    const double tr = y.x * Em.x - y.y * Em.y, ti = y.x * Em.y + y.y * Em.x;
+                    const double cr = Ek.x * tr + Ek.y * ti, ci = Ek.y * tr - Ek.x * ti;
+                    const double vm = w.V[m];
+                    H = ci; Nn = cr / vm; M = -cr; L = ci / vm;
+                } else {
+                    const double v = w.V[k], v2 = v * v, p = w.P[k], q = w.Q[k];
+                    H = -q - v2 * y.y; Nn = (p + v2 * y.x) / v; M = p - v2 * y.x; L = (q - v2 * y.y) / v;
+                }
+                if (!w.isq[m]) { Nn = 0; L = 0; }
+                if (!w.isq[k]) { M = 0; L = (k == m) ? 1.0 : 0.0; }
+                out = B4<T>{(T)H, (T)Nn, (T)M, (T)L};
+            }
+            store_b4(w.blk, s, out);
+        }
+        for (int i = tid; i < 2 * Y.n; i += blockDim.x) w.x[i] = w.g[i];
+        if (tid == 0) s_maxDx_bits = 0;
+        __syncthreads();
+        // factor, level by level
+        {
+            const int grp = tid / GL, lane = tid % GL, ng = (int)blockDim.x / GL;
+            for (int l = 0; l < Y.nlev; l++) {
+                const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+                for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, Y.lvl_nodes[s + i], lane, &s_sing);
+                __syncthreads();
+            }
+        }
+        if (s_sing) break;
+        wg_solve(Y, (const T *)w.blk, w.x);
+        for (int rs = 0; rs < J.refine; rs++) {
+            wg_residual(N, w);
+            __syncthreads();
+            wg_solve(Y, (const T *)w.blk, w.r);
+            for (int i = tid; i < 2 * Y.n; i += blockDim.x) w.x[i] += w.r[i];
+            __syncthreads();
+        }
+        // update
+        {
+            double m = 0.0;
+            for (int k = tid; k < N.nB; k += blockDim.x) {
+                if (k == N.slack) continue;
+                const int j = N.node_of_bus[k];
+                const double dth = w.x[2 * j];
+                double v = w.V[k];
+                const double th = w.th[k] + dth;
+                m = dmax_nan(m, fabs(dth));
+                if (w.isq[k]) {
+                    const double dv = w.x[2 * j + 1];
+                    v += dv;
+                    m = dmax_nan(m, fabs(dv));
+                }
+                w.V[k] = v;
+                w.th[k] = th;
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                w.E[k] = make_double2(v * cs, v * sn);
+            }
+            m = wave_max(m);
+            if ((tid & 63) == 0) atomicMax(&s_maxDx_bits, double_as_u64(m));
+        }
+        __syncthreads();
+        it++;
+    }
+    if (tid == 0) {
+        *s_it = it;
+        *s_conv = conv;
+        *s_maxg = maxG;
+    }
+    __syncthreads();
+}
+
+template <typename T, bool LDS>
+__global__ void __launch_bounds__(1024) k_batch(BatchTopo tp, BatchJob J) {
+    extern __shared__ __align__(16) char smem[];
+    __shared__ int s_item, s_it, s_conv;
+    __shared__ double s_maxg;
+    const DevNet &N = tp.N;
+    char *base = LDS ? smem : J.gws + (long long)blockIdx.x * J.ws_bytes;
+    Work<T> w = carve<T>(base, N.nB, tp.Y.n, N.nPV, tp.nslot);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(J.counter, 1);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= J.nscen) break;                         // every wave reaches this exit
+        const int sc = J.scen[item];
+        const long long off = J.off[sc];
+        const double *Pl = J.Pliq + off, *Ql = J.Qliq + off, *Qd = J.Qload + off;
+        bool warm = J.V0 != nullptr;
+        wg_nr(tp, J, w, Pl, Ql, Qd, warm ? J.V0 + off : tp.V0flat, warm ? J.th0 + off : tp.th0flat,
+              &s_it, &s_conv, &s_maxg);
+        int total_it = s_it;
+        if (warm && !s_conv) {                              // flat-start fallback
+            wg_nr(tp, J, w, Pl, Ql, Qd, tp.V0flat, tp.th0flat, &s_it, &s_conv, &s_maxg);
+            total_it += s_it;
+        }
+        for (int k = threadIdx.x; k < N.nB; k += blockDim.x) {
+            J.Vout[off + k] = w.V[k];
+            J.thout[off + k] = w.th[k];
+        }
+        if (threadIdx.x == 0) {
+            J.iters[sc] = total_it;
+            J.conv[sc] = s_conv;
+            J.maxmis[sc] = s_maxg;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_int(int *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+}  // namespace pf
+
+// ======================================================================== host
+
+namespace {
+thread_local std::string g_batch_error;
+#define BHIP(expr)                                                                 \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            g_batch_error = std::string(#expr) + ": " + hipGetErrorString(e_);     \
+            return PF_DEVICE_ERROR;                                                \
+        }                                                                          \
+    } while (0)
+#define BCHK(expr)                  \
+    do {                            \
+        pf_status s_ = (expr);      \
+        if (s_ != PF_OK) return s_; \
+    } while (0)
+constexpr long long kLdsLimit = 64 * 1024;
+}  // namespace
+
+struct BatchNet {
+    pf_network net;
+    YPattern yp;
+    Symbolic sym;
+    BatchTopo tp{};
+    bool lds = false;
+    int block = 256;
+    int grid = 0;
+    long long ws = 0;
+    std::vector<int> scen;
+    // device
+    int *d_scen = nullptr;
+    char *d_ws = nullptr;
+};
+
+struct pf_batch {
+    int device = 0;
+    pf_options opt;
+    std::vector<std::unique_ptr<BatchNet>> nets;
+    std::vector<void *> allocs;
+    hipStream_t st[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int n_scen = 0;
+    long long total = 0;
+    std::vector<long long> off;
+    std::vector<int> net_of;
+    double *d_P = nullptr, *d_Q = nullptr, *d_L = nullptr, *d_V0 = nullptr, *d_T0 = nullptr;
+    double *warm_V0 = nullptr, *warm_T0 = nullptr;
+    double *d_V = nullptr, *d_T = nullptr, *d_mm = nullptr;
+    int *d_it = nullptr, *d_cv = nullptr, *d_counter = nullptr;
+    long long *d_off = nullptr;
+    int cus = 0;
+
+    template <typename T>
+    pf_status alloc(T **p, size_t n) {
+        void *q = nullptr;
+        BHIP(hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1)));
+        allocs.push_back(q);
+        *p = (T *)q;
+        return PF_OK;
+    }
+    template <typename T>
+    pf_status upload(T **p, const std::vector<T> &v) {
+        BCHK(alloc(p, v.size()));
+        if (!v.empty()) BHIP(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        return PF_OK;
+    }
+    ~pf_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : {ev_fork, ev_mid, ev_join, t0, t1})
+            if (e) (void)hipEventDestroy(e);
+        for (void *p : allocs) (void)hipFree(p);
+        for (auto s : st)
+            if (s) (void)hipStreamDestroy(s);
+    }
+    pf_status add_net(const pf_network &net);
+    pf_status launch_all();
+};
+
+pf_status pf_batch::add_net(const pf_network &src) {
+    auto b = std::make_unique<BatchNet>();
+    b->net = src;
+    b->yp = build_ypattern(src);
+    b->sym = analyze(src, b->yp, 0);
+    const pf_network &net = b->net;
+    const YPattern &yp = b->yp;
+    const Symbolic &sym = b->sym;
+    const int nB = net.nB, n = sym.n, sl = net.slack;
+    std::vector<int> node_of_bus(nB, -1), pv_of_bus(nB, -1);
+    for (int k = 0; k < nB; k++)
+        if (k != sl) node_of_bus[k] = sym.iperm[k - (k > sl)];
+    for (int i = 0; i < net.nPV; i++) pv_of_bus[net.pv[i]] = i;
+    DevNet &N = b->tp.N;
+    DevSym &Y = b->tp.Y;
+    N.nB = nB; N.nL = net.nL; N.slack = sl; N.nPV = net.nPV; N.n = n;
+    BCHK(upload((int **)&N.rowptr, yp.rowptr));
+    BCHK(upload((int **)&N.colind, yp.colind));
+    BCHK(upload((int **)&N.coorow, yp.coorow));
+    BCHK(upload((int **)&N.ydiag, yp.diag));
+    BCHK(upload((int **)&N.yc_ptr, yp.contrib_ptr));
+    BCHK(upload((int **)&N.yc, yp.contrib));
+    BCHK(upload((double **)&N.br_r, net.r));
+    BCHK(upload((double **)&N.br_x, net.x));
+    BCHK(upload((double **)&N.br_b, net.bch));
+    BCHK(upload((double **)&N.br_tr, net.tap_re));
+    BCHK(upload((double **)&N.br_ti, net.tap_im));
+    BCHK(upload((double **)&N.gsh, net.gsh));
+    BCHK(upload((double **)&N.bsh, net.bsh));
+    BCHK(alloc(&N.y, yp.nnz));
+    BCHK(upload((int **)&N.node_of_bus, node_of_bus));
+    N.bus_of_node = nullptr;
+    BCHK(upload((int **)&N.pv_bus, net.pv));
+    BCHK(upload((int **)&N.pv_of_bus, pv_of_bus));
+    BCHK(upload((double **)&N.qmin, net.Qmin));
+    BCHK(upload((double **)&N.qmax, net.Qmax));
+    BCHK(upload((double **)&N.vset, net.Vset));
+    Y.n = n; Y.nnzL = sym.nnzL; Y.nslot = sym.nslot(); Y.nlev = sym.nlev;
+    BCHK(upload((int **)&Y.colptr, sym.colptr));
+    BCHK(upload((int **)&Y.rowind, sym.rowind));
+    Y.pcol = nullptr;
+    BCHK(upload((int **)&Y.lvl_ptr, sym.lvl_ptr));
+    BCHK(upload((int **)&Y.lvl_nodes, sym.lvl_nodes));
+    BCHK(upload((int **)&Y.fd_ptr, sym.fd_ptr));
+    BCHK(upload((int **)&Y.fc_ptr, sym.fc_ptr));
+    BCHK(upload((int **)&Y.fc_a, sym.fc_a));
+    BCHK(upload((int **)&Y.fc_b, sym.fc_b));
+    BCHK(upload((int **)&Y.rs_ptr, sym.rs_ptr));
+    BCHK(upload((int **)&Y.rs_k, sym.rs_k));
+    BCHK(upload((int **)&Y.rs_p, sym.rs_p));
+    BCHK(upload((int **)&Y.slot_e, sym.slot_e));
+    BCHK(upload((double **)&b->tp.V0flat, net.V));
+    BCHK(upload((double **)&b->tp.th0flat, net.theta));
+    b->tp.nslot = sym.nslot();
+    k_ybus<<<(yp.nnz + 255) / 256, 256, 0, st[0]>>>(N, yp.nnz);
+    BHIP(hipGetLastError());
+    BHIP(hipStreamSynchronize(st[0]));
+    const bool mixed = opt.precision != PF_FP64;
+    b->ws = mixed ? work_bytes<float>(nB, n, net.nPV, sym.nslot())
+                  : work_bytes<double>(nB, n, net.nPV, sym.nslot());
+    b->lds = b->ws <= kLdsLimit;
+    b->block = b->lds ? 256 : 1024;
+    nets.push_back(std::move(b));
+    return PF_OK;
+}
+
+pf_status pf_batch::launch_all() {
+    BHIP(hipEventRecord(ev_fork, st[0]));
+    BHIP(hipStreamWaitEvent(st[1], ev_fork, 0));
+    k_zero_int<<<1, 64, 0, st[0]>>>(d_counter, (int)nets.size());
+    BHIP(hipEventRecord(ev_mid, st[0]));
+    BHIP(hipStreamWaitEvent(st[1], ev_mid, 0));
+    const bool mixed = opt.precision != PF_FP64;
+    for (size_t i = 0; i < nets.size(); i++) {
+        BatchNet &b = *nets[i];
+        if (b.scen.empty()) continue;
+        BatchJob J{};
+        J.nscen = (int)b.scen.size();
+        J.scen = b.d_scen;
+        J.off = d_off;
+        J.Pliq = d_P; J.Qliq = d_Q; J.Qload = d_L;
+        J.V0 = d_V0; J.th0 = d_T0;
+        J.Vout = d_V; J.thout = d_T;
+        J.iters = d_it; J.conv = d_cv; J.maxmis = d_mm;
+        J.counter = d_counter + i;
+        J.gws = b.d_ws;
+        J.ws_bytes = b.ws;
+        J.max_iter = opt.no_max_iter;
+        J.lim = opt.lim_inj_reat ? 1 : 0;
+        J.refine = mixed ? opt.refine_steps : 0;
+        J.tol = opt.tolerancia;
+        J.diverge = 1e6;
+        hipStream_t s = st[i & 1];
+        size_t shm = b.lds ? (size_t)b.ws : 0;
+        if (mixed) {
+            if (b.lds) k_batch<float, true><<<b.grid, b.block, shm, s>>>(b.tp, J);
+            else k_batch<float, false><<<b.grid, b.block, 0, s>>>(b.tp, J);
+        } else {
+            if (b.lds) k_batch<double, true><<<b.grid, b.block, shm, s>>>(b.tp, J);
+            else k_batch<double, false><<<b.grid, b.block, 0, s>>>(b.tp, J);
+        }
+        BHIP(hipGetLastError());
+    }
+    BHIP(hipEventRecord(ev_join, st[1]));
+    BHIP(hipStreamWaitEvent(st[0], ev_join, 0));
+    return PF_OK;
+}
+
+extern "C" {
+
+pf_status pf_batch_create(const pf_network *const *nets, int n_nets, const pf_options *opt, int device,
+                          pf_batch **out) {
+    if (!nets || n_nets <= 0 || n_nets > 64 || !opt || !out) return PF_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PF_DEVICE_ERROR;
+    auto b = std::make_unique<pf_batch>();
+    b->device = device;
+    b->opt = *opt;
+    BHIP(hipSetDevice(device));
+    for (auto &s : b->st) BHIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    BHIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    BHIP(hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming));
+    BHIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+    BHIP(hipEventCreate(&b->t0));
+    BHIP(hipEventCreate(&b->t1));
+    hipDeviceProp_t prop;
+    BHIP(hipGetDeviceProperties(&prop, device));
+    b->cus = prop.multiProcessorCount;
+    for (int i = 0; i < n_nets; i++) {
+        if (!nets[i] || nets[i]->nB < 2 || nets[i]->slack < 0) return PF_INVALID_ARG;
+        BCHK(b->add_net(*nets[i]));
+    }
+    BCHK(b->alloc(&b->d_counter, 64));
+    *out = b.release();
+    return PF_OK;
+}
+
+void pf_batch_destroy(pf_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    delete b;
+}
+
+pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, const double *Pliq,
+                                 const double *Qliq, const double *Qload, const double *V0,
+                                 const double *theta0) {
+    if (!b || n_scen <= 0 || !net_of || !Pliq || !Qliq || !Qload || ((V0 == nullptr) != (theta0 == nullptr)))
+        return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    const bool first = b->n_scen == 0;
+    if (!first) {
+        // a later call refreshes the data of the same scenario layout
+        if (n_scen != b->n_scen || !std::equal(net_of, net_of + n_scen, b->net_of.begin()))
+            return PF_INVALID_ARG;
+    } else {
+        b->n_scen = n_scen;
+        b->net_of.assign(net_of, net_of + n_scen);
+        b->off.resize(n_scen);
+        long long tot = 0;
+        for (auto &nb : b->nets) nb->scen.clear();
+        for (int s = 0; s < n_scen; s++) {
+            if (net_of[s] < 0 || net_of[s] >= (int)b->nets.size()) {
+                b->n_scen = 0;
+                return PF_INVALID_ARG;
+            }
+            b->off[s] = tot;
+            tot += b->nets[net_of[s]]->net.nB;
+            b->nets[net_of[s]]->scen.push_back(s);
+        }
+        b->total = tot;
+        BCHK(b->alloc(&b->d_P, tot));
+        BCHK(b->alloc(&b->d_Q, tot));
+        BCHK(b->alloc(&b->d_L, tot));
+        BCHK(b->alloc(&b->d_V, tot));
+        BCHK(b->alloc(&b->d_T, tot));
+        BCHK(b->alloc(&b->d_mm, n_scen));
+        BCHK(b->alloc(&b->d_it, n_scen));
+        BCHK(b->alloc(&b->d_cv, n_scen));
+        BCHK(b->upload(&b->d_off, b->off));
+        const bool mixed = b->opt.precision != PF_FP64;
+        for (auto &nbp : b->nets) {
+            BatchNet &nb = *nbp;
+            if (nb.scen.empty()) continue;
+            BCHK(b->upload(&nb.d_scen, nb.scen));
+            int per_cu = 1;
+            if (nb.lds) {
+                hipError_t e;
+                if (mixed) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<float, true>, nb.block, nb.ws);
+                else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<double, true>, nb.block, nb.ws);
+                if (e != hipSuccess || per_cu < 1) per_cu = 1;
+            }
+            nb.grid = std::max(1, std::min<int>((int)nb.scen.size(), per_cu * b->cus));
+            if (!nb.lds) BCHK(b->alloc(&nb.d_ws, (size_t)nb.grid * nb.ws));
+        }
+    }
+    const size_t tot = (size_t)b->total;
+    BHIP(hipMemcpy(b->d_P, Pliq, sizeof(double) * tot, hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->d_Q, Qliq, sizeof(double) * tot, hipMemcpyHostToDevice));
+    BHIP(hipMemcpy(b->d_L, Qload, sizeof(double) * tot, hipMemcpyHostToDevice));
+    const bool had_warm = b->d_V0 != nullptr;
+    if (V0) {
+        if (!b->warm_V0) {
+            BCHK(b->alloc(&b->warm_V0, tot));
+            BCHK(b->alloc(&b->warm_T0, tot));
+        }
+        BHIP(hipMemcpy(b->warm_V0, V0, sizeof(double) * tot, hipMemcpyHostToDevice));
+        BHIP(hipMemcpy(b->warm_T0, theta0, sizeof(double) * tot, hipMemcpyHostToDevice));
+        b->d_V0 = b->warm_V0;
+        b->d_T0 = b->warm_T0;
+    } else {
+        b->d_V0 = b->d_T0 = nullptr;
+    }
+    if (had_warm != (b->d_V0 != nullptr) && b->gexec) {   // kernel arguments change: capture again
+        (void)hipGraphExecDestroy(b->gexec);
+        (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    return PF_OK;
+}
+
+pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms) {
+    if (!b || !b->n_scen) return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    if (!b->gexec) {
+        BHIP(hipStreamBeginCapture(b->st[0], hipStreamCaptureModeThreadLocal));
+        pf_status s = b->launch_all();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->st[0], &g);
+        if (s != PF_OK) return s;
+        BHIP(e);
+        b->graph = g;
+        BHIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    BHIP(hipEventRecord(b->t0, b->st[0]));
+    BHIP(hipGraphLaunch(b->gexec, b->st[0]));
+    BHIP(hipEventRecord(b->t1, b->st[0]));
+    BHIP(hipEventSynchronize(b->t1));
+    float ms = 0;
+    BHIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    return PF_OK;
+}
+
+pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations, int *converged,
+                       double *max_mismatch) {
+    if (!b || !b->n_scen) return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st[0]));
+    if (V) BHIP(hipMemcpy(V, b->d_V, sizeof(double) * b->total, hipMemcpyDeviceToHost));
+    if (theta) BHIP(hipMemcpy(theta, b->d_T, sizeof(double) * b->total, hipMemcpyDeviceToHost));
+    if (iterations) BHIP(hipMemcpy(iterations, b->d_it, sizeof(int) * b->n_scen, hipMemcpyDeviceToHost));
+    if (converged) BHIP(hipMemcpy(converged, b->d_cv, sizeof(int) * b->n_scen, hipMemcpyDeviceToHost));
+    if (max_mismatch) BHIP(hipMemcpy(max_mismatch, b->d_mm, sizeof(double) * b->n_scen, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+}  // extern "C"

@@ -57,7 +57,7 @@ struct DevState {
 
 // Symbolic factor program on the device.
 struct DevSym {
-    int n, nnzL, nslot;
+    int n, nnzL, nslot, nlev;
     const int *colptr, *rowind, *pcol;      // pcol[p] = column of L position p
     const int *lvl_ptr, *lvl_nodes;
     const int *fd_ptr, *fc_ptr, *fc_a, *fc_b;

@@ -47,6 +47,7 @@ __device__ __forceinline__ B4<T> mul(const B4<T> &a, const B4<T> &b) {
 // Smith's complex division (CACM Algorithm 116, 1962): (ar + j ai) / (br + j bi)
 __device__ __forceinline__ void smith_div(double ar, double ai, double br, double bi, double &cr,
                                           double &ci) {
+#pragma clang fp contract(off)
     if (fabs(br) >= fabs(bi)) {
         double r = bi / br, d = br + bi * r;
         cr = (ar + ai * r) / d;

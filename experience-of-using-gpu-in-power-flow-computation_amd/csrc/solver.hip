@@ -198,6 +198,7 @@ pf_status pf_solver::build() {
     dy.n = n;
     dy.nnzL = sym.nnzL;
     dy.nslot = sym.nslot();
+    dy.nlev = sym.nlev;
     PFCHK(upload((int **)&dy.colptr, sym.colptr));
     PFCHK(upload((int **)&dy.rowind, sym.rowind));
     PFCHK(upload((int **)&dy.pcol, pcol));

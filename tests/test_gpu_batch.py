@@ -1,0 +1,122 @@
+"""GPU parity of the batched persistent solver (one workgroup per scenario)
+against the oracle solving each scenario on its own, and of its warm start /
+flat-start fallback.  Tolerance: |dV| <= 1e-6 pu, |dtheta| <= 1e-6 rad."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-batch]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _scenarios(synth, net, n, seed0):
+    """Perturbed injections: per-bus load factors clip(exp(0.15 N), 0.7, 1.3),
+    PV dispatch scaled by the ratio of total loads."""
+    is_pv = np.zeros(net.nB, bool)
+    is_pv[net.pv] = True
+    P, Q, L = [], [], []
+    for s in range(n):
+        p, q, l = synth.scenario_injections(net.Pg, net.Pload, net.Qg, net.Qload, is_pv, seed0 + s)
+        P.append(p)
+        Q.append(q)
+        L.append(l)
+    return np.array(P), np.array(Q), np.array(L)
+
+
+def _oracle_solve(oc, P, Q, L, lim=True):
+    out = []
+    for s in range(len(P)):
+        out.append(oc.solve(tol=1e-8, lim=lim, Pliq=P[s], Qliq=Q[s], Qload=L[s], reuse_symbolic=True))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_uneven(pf, oracle, synth, cases, precision):
+    p118 = os.path.join(GOLDEN, "c118.cdf")
+    p944 = cases.path("c944")
+    n118, n944 = pf.Network(p118), pf.Network(p944)
+    o118, o944 = oracle.Case(p118), oracle.Case(p944)
+    A = _scenarios(synth, n118, 24, 3000000)
+    B = _scenarios(synth, n944, 4, 3100000)
+    net_of = np.array([0] * 24 + [1] * 4, np.int32)
+    order = np.random.default_rng(0).permutation(len(net_of))   # interleave the two sizes
+    net_of = net_of[order]
+    rows = {0: list(range(24)), 1: list(range(4))}
+    P, Q, L = [], [], []
+    idx = {0: 0, 1: 0}
+    src = {0: A, 1: B}
+    picked = []
+    for k in net_of:
+        i = rows[k][idx[k]]
+        idx[k] += 1
+        picked.append((k, i))
+        P.append(src[k][0][i]); Q.append(src[k][1][i]); L.append(src[k][2][i])
+    opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1,
+                      precision=pf.MIXED if precision == "mixed" else pf.FP64)
+    _say("batch", precision, len(net_of))
+    b = pf.Batch([n118, n944], opts)
+    b.set_scenarios(net_of, np.concatenate(P), np.concatenate(Q), np.concatenate(L))
+    ms = b.solve()
+    V, th, it, cv, mm = b.results()
+    _say("elapsed ms", ms, "iters", it.tolist())
+    off = 0
+    for s, (k, i) in enumerate(picked):
+        nb = (n118, n944)[k].nB
+        oc = (o118, o944)[k]
+        r = oc.solve(tol=1e-8, lim=True, Pliq=src[k][0][i], Qliq=src[k][1][i], Qload=src[k][2][i],
+                     reuse_symbolic=True)
+        assert r["converged"]
+        assert cv[s], (s, k, i, it[s], mm[s])
+        assert mm[s] <= 1e-8
+        assert np.abs(V[off:off + nb] - r["V"]).max() <= 1e-6
+        assert np.abs(th[off:off + nb] - r["theta"]).max() <= 1e-6
+        if precision == "fp64":
+            assert it[s] == r["iterations"]
+        off += nb
+    # replaying the captured graph gives the same answer
+    b.solve()
+    V2, th2, it2, cv2, _ = b.results()
+    assert np.array_equal(V, V2) and np.array_equal(it, it2)
+
+
+def test_batch_warm_start_and_fallback(pf, oracle, synth):
+    p118 = os.path.join(GOLDEN, "c118.cdf")
+    net = pf.Network(p118)
+    oc = oracle.Case(p118)
+    P, Q, L = _scenarios(synth, net, 8, 3200000)
+    ref = _oracle_solve(oc, P, Q, L)
+    opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, precision=pf.MIXED)
+    b = pf.Batch([net], opts)
+    net_of = np.zeros(8, np.int32)
+    b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel())
+    b.solve()
+    _, _, it_flat, cv_flat, _ = b.results()
+    assert cv_flat.all()
+    # warm start at the oracle's answer (PV buses restart at their setpoints and
+    # re-switch): fewer iterations than the flat start, same answer
+    V0 = np.concatenate([r["V"] for r in ref])
+    T0 = np.concatenate([r["theta"] for r in ref])
+    b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel(), V0, T0)
+    b.solve()
+    V, th, it, cv, mm = b.results()
+    assert cv.all() and (mm <= 1e-8).all()
+    assert it.mean() < it_flat.mean()
+    for s in range(8):
+        assert np.abs(V[118 * s:118 * (s + 1)] - ref[s]["V"]).max() <= 1e-6
+    # a useless start (|V| = 0.05, random angles) diverges and falls back to flat start
+    bad_V = np.full_like(V0, 0.05)
+    bad_T = np.random.default_rng(1).uniform(-3, 3, T0.shape)
+    b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel(), bad_V, bad_T)
+    b.solve()
+    V, th, it, cv, mm = b.results()
+    assert cv.all()
+    for s in range(8):
+        assert np.abs(V[118 * s:118 * (s + 1)] - ref[s]["V"]).max() <= 1e-6

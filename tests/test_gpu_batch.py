@@ -100,17 +100,30 @@ def test_batch_warm_start_and_fallback(pf, oracle, synth):
     b.solve()
     _, _, it_flat, cv_flat, _ = b.results()
     assert cv_flat.all()
-    # warm start at the oracle's answer (PV buses restart at their setpoints and
-    # re-switch): fewer iterations than the flat start, same answer
-    V0 = np.concatenate([r["V"] for r in ref])
-    T0 = np.concatenate([r["theta"] for r in ref])
+    # warm start near the oracle's answer: fewer iterations than the flat start.
+    # The start sets |V| of PQ buses and all non-slack angles; PV buses begin at
+    # their setpoints.  Back-switched buses keep the |V| they reached while
+    # limited (CPU/limInjReat.cpp:62-87), so the answer depends on the start:
+    # compare with the oracle started from the same state.
+    rng = np.random.default_rng(9)
+    V0 = np.concatenate([r["V"] for r in ref]) + 1e-3 * rng.standard_normal(8 * 118)
+    T0 = np.concatenate([r["theta"] for r in ref]) + 1e-3 * rng.standard_normal(8 * 118)
     b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel(), V0, T0)
     b.solve()
     V, th, it, cv, mm = b.results()
     assert cv.all() and (mm <= 1e-8).all()
     assert it.mean() < it_flat.mean()
+    is_pq = np.zeros(118, bool)
+    is_pq[net.pq] = True
     for s in range(8):
-        assert np.abs(V[118 * s:118 * (s + 1)] - ref[s]["V"]).max() <= 1e-6
+        v0 = np.where(is_pq, V0[118 * s:118 * (s + 1)], net.V)
+        t0 = T0[118 * s:118 * (s + 1)].copy()
+        t0[net.slack] = net.theta[net.slack]
+        r = oc.solve(tol=1e-8, lim=True, Pliq=P[s], Qliq=Q[s], Qload=L[s], V0=v0, theta0=t0,
+                     reuse_symbolic=True)
+        assert r["converged"]
+        assert np.abs(V[118 * s:118 * (s + 1)] - r["V"]).max() <= 1e-6
+        assert np.abs(th[118 * s:118 * (s + 1)] - r["theta"]).max() <= 1e-6
     # a useless start (|V| = 0.05, random angles) diverges and falls back to flat start
     bad_V = np.full_like(V0, 0.05)
     bad_T = np.random.default_rng(1).uniform(-3, 3, T0.shape)

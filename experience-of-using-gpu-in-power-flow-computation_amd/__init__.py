@@ -99,6 +99,19 @@ def lib():
         L.pf_probe_free.argtypes = [P]
         L.pf_probe_get.argtypes = [P, P, P, P, P, P]
         L.pf_probe_factor_solve.argtypes = [P, P, P]
+        if hasattr(L, "pf_warmstart_create"):
+            L.pf_warmstart_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int,
+                                              C.POINTER(P)]
+            L.pf_warmstart_destroy.argtypes = [P]
+            L.pf_warmstart_info.argtypes = [P, P, P]
+            L.pf_warmstart_set_norm.argtypes = [P, P, P, P, P]
+            L.pf_warmstart_get_params.argtypes = [P, P]
+            L.pf_warmstart_set_params.argtypes = [P, P]
+            L.pf_warmstart_grad.argtypes = [P, P, P, C.c_int, P, P]
+            L.pf_warmstart_fit.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                           C.c_ulonglong, P]
+            L.pf_warmstart_predict.argtypes = [P, P, C.c_int, P]
+            L.pf_warmstart_gemm_probe.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
         if hasattr(L, "pf_batch_create"):
             L.pf_batch_create.argtypes = [P, C.c_int, C.POINTER(Options), C.c_int, C.POINTER(P)]
             L.pf_batch_destroy.argtypes = [P]
@@ -355,3 +368,77 @@ class Batch:
         mm = np.zeros(self.n_scen)
         _chk(lib().pf_batch_get(self.h, _p(V), _p(th), _p(it), _p(cv), _p(mm)), "pf_batch_get")
         return V, th, it, cv.astype(bool), mm
+
+
+class WarmStart:
+    """The learned warm start (MLP) trained and run by libpf's own kernels."""
+
+    def __init__(self, n_in: int, n_out: int, hidden: int = 512, n_hidden_layers: int = 3,
+                 seed: int = 1, device: int = 0):
+        h = C.c_void_p()
+        _chk(lib().pf_warmstart_create(n_in, n_out, hidden, n_hidden_layers, seed, device, C.byref(h)),
+             "pf_warmstart_create")
+        self.h, self.n_in, self.n_out = h, n_in, n_out
+        n = C.c_size_t()
+        nl = C.c_int()
+        lib().pf_warmstart_info(h, C.byref(n), C.byref(nl))
+        self.n_params, self.n_layers = n.value, nl.value
+        self.hidden, self.n_hidden_layers = hidden, n_hidden_layers
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.pf_warmstart_destroy(self.h)
+            self.h = None
+
+    def layer_shapes(self):
+        K = [self.n_in] + [self.hidden] * self.n_hidden_layers
+        N = [self.hidden] * self.n_hidden_layers + [self.n_out]
+        return list(zip(K, N))
+
+    def set_norm(self, xm, xs, ym, ys):
+        a = [np.ascontiguousarray(v, np.float32) for v in (xm, xs, ym, ys)]
+        _chk(lib().pf_warmstart_set_norm(self.h, *[_p(v) for v in a]), "pf_warmstart_set_norm")
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        _chk(lib().pf_warmstart_get_params(self.h, _p(out)), "pf_warmstart_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _chk(lib().pf_warmstart_set_params(self.h, _p(p)), "pf_warmstart_set_params")
+
+    def grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _chk(lib().pf_warmstart_grad(self.h, _p(X), _p(Y), X.shape[0], _p(g), C.byref(loss)),
+             "pf_warmstart_grad")
+        return g, loss.value
+
+    def fit(self, X, Y, epochs=10, batch=256, lr=1e-3, weight_decay=1e-4, seed=7):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        el = np.zeros(epochs)
+        _chk(lib().pf_warmstart_fit(self.h, _p(X), _p(Y), X.shape[0], epochs, batch, lr, weight_decay, seed,
+                                    _p(el)), "pf_warmstart_fit")
+        return el
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], self.n_out), np.float32)
+        _chk(lib().pf_warmstart_predict(self.h, _p(X), X.shape[0], _p(Y)), "pf_warmstart_predict")
+        return Y
+
+    def gemm_probe(self, A, B, ta=False, tb=False):
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        M = A.shape[1] if ta else A.shape[0]
+        K = A.shape[0] if ta else A.shape[1]
+        N = B.shape[0] if tb else B.shape[1]
+        Cm = np.zeros((M, N), np.float32)
+        _chk(lib().pf_warmstart_gemm_probe(self.h, int(ta), int(tb), M, N, K, _p(A), _p(B), _p(Cm)),
+             "pf_warmstart_gemm_probe")
+        return Cm

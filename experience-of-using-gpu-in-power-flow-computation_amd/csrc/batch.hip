@@ -428,11 +428,17 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
         if (tid == 0) s_maxDx_bits = 0;
         __syncthreads();
         // factor, level by level
+        // wide levels: a lane group per node; narrow levels: the whole
+        // workgroup on the level's destinations (coop_factor_level)
         {
             const int grp = tid / GL, lane = tid % GL, ng = (int)blockDim.x / GL;
             for (int l = 0; l < Y.nlev; l++) {
                 const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
-                for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, Y.lvl_nodes[s + i], lane, &s_sing);
+                if (c >= ng) {
+                    for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, Y.lvl_nodes[s + i], lane, &s_sing);
+                } else {
+                    coop_factor_level<T, GL>(Y, w.blk, l, &s_sing);
+                }
                 __syncthreads();
             }
         }
@@ -658,6 +664,9 @@ pf_status pf_batch::add_net(const pf_network &src) {
     BCHK(upload((int **)&Y.rs_k, sym.rs_k));
     BCHK(upload((int **)&Y.rs_p, sym.rs_p));
     BCHK(upload((int **)&Y.slot_e, sym.slot_e));
+    BCHK(upload((int **)&Y.ld_ptr, sym.ld_ptr));
+    BCHK(upload((int **)&Y.ld_node, sym.ld_node));
+    BCHK(upload((int **)&Y.ld_d, sym.ld_d));
     BCHK(upload((double **)&b->tp.V0flat, net.V));
     BCHK(upload((double **)&b->tp.th0flat, net.theta));
     b->tp.nslot = sym.nslot();

@@ -331,10 +331,8 @@ __global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
     for (int l = lv0; l < lv1; l++) {
-        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
-        for (int idx = w; idx < c; idx += 16) factor_node(Y, blk, Y.lvl_nodes[s + idx], lane, &ctl->singular);
+        coop_factor_level<T, 16>(Y, blk, l, &ctl->singular);
         __syncthreads();
     }
 }
@@ -402,13 +400,30 @@ __global__ void __launch_bounds__(256) k_bwd_level(DevSym Y, const T *blk, doubl
     bwd_node(Y, blk, x, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 15);
 }
 
+// narrow levels: one wavefront per node
 template <typename T>
 __global__ void __launch_bounds__(1024) k_fwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    const int g = (int)threadIdx.x / 16, sl = (int)threadIdx.x & 15;
+    const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
     for (int l = lv0; l < lv1; l++) {
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
-        for (int idx = g; idx < c; idx += 64) fwd_node(Y, blk, x, Y.lvl_nodes[s + idx], sl);
+        for (int idx = w; idx < c; idx += 16) {
+            const int j = Y.lvl_nodes[s + idx];
+            double a0 = 0, a1 = 0;
+            for (int q = Y.rs_ptr[j] + lane; q < Y.rs_ptr[j + 1]; q += 64) {
+                B4<T> L = load_b4(blk, Y.n + Y.rs_p[q]);
+                const int k = Y.rs_k[q];
+                const double x0 = x[2 * k], x1 = x[2 * k + 1];
+                a0 += (double)L.a * x0 + (double)L.b * x1;
+                a1 += (double)L.c * x0 + (double)L.d * x1;
+            }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            if (lane == 0) {
+                x[2 * j] -= a0;
+                x[2 * j + 1] -= a1;
+            }
+        }
         __syncthreads();
     }
 }
@@ -416,10 +431,29 @@ __global__ void __launch_bounds__(1024) k_fwd_tail(DevSym Y, const T *blk, doubl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    const int g = (int)threadIdx.x / 16, sl = (int)threadIdx.x & 15;
+    const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
+    const int U0 = Y.n + Y.nnzL;
     for (int l = lv1 - 1; l >= lv0; l--) {
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
-        for (int idx = g; idx < c; idx += 64) bwd_node(Y, blk, x, Y.lvl_nodes[s + idx], sl);
+        for (int idx = w; idx < c; idx += 16) {
+            const int j = Y.lvl_nodes[s + idx];
+            double a0 = 0, a1 = 0;
+            for (int p = Y.colptr[j] + lane; p < Y.colptr[j + 1]; p += 64) {
+                B4<T> U = load_b4(blk, U0 + p);
+                const int i = Y.rowind[p];
+                const double x0 = x[2 * i], x1 = x[2 * i + 1];
+                a0 += (double)U.a * x0 + (double)U.b * x1;
+                a1 += (double)U.c * x0 + (double)U.d * x1;
+            }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            if (lane == 0) {
+                const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
+                B4<T> D = load_b4(blk, j);
+                x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
+                x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
+            }
+        }
         __syncthreads();
     }
 }

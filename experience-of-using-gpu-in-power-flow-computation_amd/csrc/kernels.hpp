@@ -63,6 +63,7 @@ struct DevSym {
     const int *fd_ptr, *fc_ptr, *fc_a, *fc_b;
     const int *rs_ptr, *rs_k, *rs_p;
     const int *slot_e;
+    const int *ld_ptr, *ld_node, *ld_d;     // flattened destinations per level
 };
 
 }  // namespace pf

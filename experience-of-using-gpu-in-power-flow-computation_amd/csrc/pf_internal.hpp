@@ -53,6 +53,10 @@ struct Symbolic {
     std::vector<int> fd_ptr;    // [n+1] into destination list
     std::vector<int> fc_ptr;    // [ndest+1] into contribution pairs
     std::vector<int> fc_a, fc_b;    // pairs (p_ik, p_jk): positions in L/U arrays
+    // destinations of every level, flattened (cooperative factorisation):
+    // entry e of level l (ld_ptr[l] <= e < ld_ptr[l+1]) is destination
+    // ld_d[e] (an index into fc_ptr) of node ld_node[e]
+    std::vector<int> ld_ptr, ld_node, ld_d;
     // forward-solve rows: per node j, k in rowstruct(j) and position p_jk
     std::vector<int> rs_ptr, rs_k, rs_p;
     // Jacobian slot map: block slot -> Y entry (-1: fill-in, zero)

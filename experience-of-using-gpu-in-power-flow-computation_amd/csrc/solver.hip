@@ -212,6 +212,9 @@ pf_status pf_solver::build() {
     PFCHK(upload((int **)&dy.rs_k, sym.rs_k));
     PFCHK(upload((int **)&dy.rs_p, sym.rs_p));
     PFCHK(upload((int **)&dy.slot_e, sym.slot_e));
+    PFCHK(upload((int **)&dy.ld_ptr, sym.ld_ptr));
+    PFCHK(upload((int **)&dy.ld_node, sym.ld_node));
+    PFCHK(upload((int **)&dy.ld_d, sym.ld_d));
     PFCHK(alloc(&blk64, 4 * (size_t)sym.nslot()));
     if (opt.precision == PF_MIXED) PFCHK(alloc(&blk32, 4 * (size_t)sym.nslot()));
 

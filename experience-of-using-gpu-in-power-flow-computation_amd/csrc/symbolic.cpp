@@ -244,6 +244,19 @@ Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width) {
             s.fc_ptr.push_back((int)s.fc_a.size());
         }
     }
+    // flattened destinations per level
+    s.ld_ptr.assign(s.nlev + 1, 0);
+    for (int l = 0; l < s.nlev; l++) {
+        s.ld_ptr[l + 1] = s.ld_ptr[l];
+        for (int q = s.lvl_ptr[l]; q < s.lvl_ptr[l + 1]; q++) {
+            int j = s.lvl_nodes[q];
+            for (int d = s.fd_ptr[j]; d < s.fd_ptr[j + 1]; d++) {
+                s.ld_node.push_back(j);
+                s.ld_d.push_back(d);
+            }
+            s.ld_ptr[l + 1] += s.fd_ptr[j + 1] - s.fd_ptr[j];
+        }
+    }
     // Jacobian slot map
     s.slot_e.assign(s.nslot(), -1);
     for (int k = 0; k < nB; k++) {

@@ -499,10 +499,32 @@ def write_cdf(case: Case, path: str, wide: bool | None = None) -> None:
 # Scenario perturbations (batched workloads, warm-start training data)
 # --------------------------------------------------------------------------
 
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def _mix64(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def load_factors_batch(n_bus: int, seeds) -> np.ndarray:
+    """Per-bus load factors of many scenarios at once: row s is the factor
+    vector of SplitMix64 stream ``seeds[s]`` (k-th output = mix(seed + k*gamma)),
+    two uniforms per bus through Box-Muller, clip(exp(0.15 N(0,1)), 0.7, 1.3)."""
+    seeds = np.asarray(seeds, dtype=np.uint64).reshape(-1, 1)
+    k = np.arange(1, 2 * n_bus + 1, dtype=np.uint64).reshape(1, -1)
+    with np.errstate(over="ignore"):
+        z = _mix64(seeds + k * _GAMMA)
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u1, u2 = u[:, 0::2], u[:, 1::2]
+    nrm = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+    return np.clip(np.exp(0.15 * nrm), 0.7, 1.3)
+
+
 def load_factors(n_bus: int, seed: int) -> np.ndarray:
     """clip(exp(0.15 * N(0,1)), 0.7, 1.3) per bus for one scenario."""
-    rng = SplitMix64(seed)
-    return np.array([min(1.3, max(0.7, math.exp(0.15 * rng.normal()))) for _ in range(n_bus)])
+    return load_factors_batch(n_bus, [seed])[0]
 
 
 def scenario_injections(pg: np.ndarray, pload: np.ndarray, qg: np.ndarray, qload: np.ndarray,
@@ -513,9 +535,16 @@ def scenario_injections(pg: np.ndarray, pload: np.ndarray, qg: np.ndarray, qload
     scaled by the ratio of the new to the old total active load.
     Returns (Pliq, Qliq, Qload') arrays over buses.
     """
-    f = load_factors(len(pload), seed)
-    pl = pload * f
-    ql = qload * f
-    ratio = pl.sum() / pload.sum() if pload.sum() != 0 else 1.0
-    pg2 = np.where(is_pv, pg * ratio, pg)
-    return pg2 - pl, qg - ql, ql
+    P, Q, L = scenario_batch(pg, pload, qg, qload, is_pv, [seed])
+    return P[0], Q[0], L[0]
+
+
+def scenario_batch(pg, pload, qg, qload, is_pv, seeds):
+    """Vectorised :func:`scenario_injections` over many seeds -> (Pliq, Qliq, Qload'), each [n, nB]."""
+    f = load_factors_batch(len(pload), seeds)
+    pl = pload[None, :] * f
+    ql = qload[None, :] * f
+    tot = pload.sum()
+    ratio = pl.sum(1, keepdims=True) / tot if tot != 0 else np.ones((len(f), 1))
+    pg2 = np.where(is_pv[None, :], pg[None, :] * ratio, pg[None, :])
+    return pg2 - pl, qg[None, :] - ql, ql

@@ -34,6 +34,8 @@
 #ifndef PF_H
 #define PF_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -199,6 +201,32 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of,
 pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms);
 pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations,
                        int *converged, double *max_mismatch);
+
+/* ------------------------------------------------------------ warm start */
+/* MLP: injections + setpoints -> |V|, theta.  BF16 weights/activations,
+ * FP32 accumulation on the matrix cores, FP32 master weights and AdamW.
+ * Parameter layout: per layer W [K x N] row-major then bias [N].           */
+typedef struct pf_warmstart pf_warmstart;
+pf_status pf_warmstart_create(int n_in, int n_out, int hidden, int n_hidden_layers,
+                              unsigned long long seed, int device, pf_warmstart **out);
+void pf_warmstart_destroy(pf_warmstart *w);
+pf_status pf_warmstart_info(pf_warmstart *w, size_t *n_params, int *n_layers);
+/* input/target normalisation (x' = (x - mean) / std; std <= 0 means 1)     */
+pf_status pf_warmstart_set_norm(pf_warmstart *w, const float *x_mean, const float *x_std,
+                                const float *y_mean, const float *y_std);
+pf_status pf_warmstart_get_params(pf_warmstart *w, float *out);
+pf_status pf_warmstart_set_params(pf_warmstart *w, const float *in);
+/* forward + backward of the mean squared error on normalised targets       */
+pf_status pf_warmstart_grad(pf_warmstart *w, const float *X, const float *Y, int B,
+                            float *grads, double *loss);
+/* AdamW over a resident data set (n rows), minibatches of B (B % 8 == 0)   */
+pf_status pf_warmstart_fit(pf_warmstart *w, const float *X, const float *Y, int n, int epochs,
+                           int B, float lr, float weight_decay, unsigned long long seed,
+                           double *epoch_loss);
+pf_status pf_warmstart_predict(pf_warmstart *w, const float *X, int B, float *Y);
+/* one product through the training GEMM kernel: C = op(A) op(B)            */
+pf_status pf_warmstart_gemm_probe(pf_warmstart *w, int ta, int tb, int M, int N, int K,
+                                  const float *A, const float *B, float *C);
 
 #ifdef __cplusplus
 }

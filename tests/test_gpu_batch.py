@@ -22,13 +22,7 @@ def _scenarios(synth, net, n, seed0):
     PV dispatch scaled by the ratio of total loads."""
     is_pv = np.zeros(net.nB, bool)
     is_pv[net.pv] = True
-    P, Q, L = [], [], []
-    for s in range(n):
-        p, q, l = synth.scenario_injections(net.Pg, net.Pload, net.Qg, net.Qload, is_pv, seed0 + s)
-        P.append(p)
-        Q.append(q)
-        L.append(l)
-    return np.array(P), np.array(Q), np.array(L)
+    return synth.scenario_batch(net.Pg, net.Pload, net.Qg, net.Qload, is_pv, np.arange(seed0, seed0 + n))
 
 
 def _oracle_solve(oc, P, Q, L, lim=True):

@@ -8,7 +8,11 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, IEEE, csr_to_dense, power, ybus_dense
+import sys
+
+from conftest import GOLDEN, IEEE, ROOT, csr_to_dense, power, ybus_dense
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 TOL_NR = 1e-8
 
@@ -328,3 +332,35 @@ def test_limit_rules_known_answers(oracle):
     # undo never switches a free bus, whatever its Q
     lq4, _, f4 = c.limits(1, Q, V, np.zeros(n, bool), QL)
     assert f4 == 0 and not lq4.any()
+
+
+# ------------------------------------------------------ warm-start MLP oracle
+
+def test_mlp_oracle_gradient_fd():
+    """The MLP oracle's backward pass against central differences (FP64)."""
+    import mlp_oracle as M
+    rng = np.random.default_rng(0)
+    shapes = [(16, 24), (24, 24), (24, 8)]
+    n = sum(k * m + m for k, m in shapes)
+    p = rng.standard_normal(n) * 0.3
+    X = rng.standard_normal((8, 16))
+    T = rng.standard_normal((8, 8))
+    xm, xs = X.mean(0), X.std(0) + 0.5
+    ym, ys = T.mean(0), T.std(0) + 0.5
+    loss, g = M.loss_and_grad(p, shapes, X, T, xm, xs, ym, ys, quantize=False)
+    h = 1e-6
+    for i in rng.choice(n, 40, replace=False):
+        e = np.zeros(n)
+        e[i] = h
+        lp, _ = M.loss_and_grad(p + e, shapes, X, T, xm, xs, ym, ys, quantize=False)
+        lm, _ = M.loss_and_grad(p - e, shapes, X, T, xm, xs, ym, ys, quantize=False)
+        assert abs((lp - lm) / (2 * h) - g[i]) <= 1e-6 * (1 + abs(g[i])), i
+
+
+def test_bf16_rounding_known_answers():
+    import mlp_oracle as M
+    # 1 + 2^-8 is exactly halfway between bf16 neighbours 1 and 1 + 2^-7: ties to even -> 1
+    assert M.bf16(np.float32(1 + 2 ** -8)) == 1.0
+    assert M.bf16(np.float32(1 + 3 * 2 ** -8)) == 1 + 2 ** -6
+    assert M.bf16(np.float32(1 + 2 ** -7)) == 1 + 2 ** -7
+    assert M.bf16(np.float32(-3.0)) == -3.0

@@ -99,6 +99,7 @@ def lib():
         L.pf_probe_free.argtypes = [P]
         L.pf_probe_get.argtypes = [P, P, P, P, P, P]
         L.pf_probe_factor_solve.argtypes = [P, P, P]
+        L.pf_probe_levels.argtypes = [P, P, P]
         if hasattr(L, "pf_warmstart_create"):
             L.pf_warmstart_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int,
                                               C.POINTER(P)]
@@ -201,6 +202,9 @@ class SymbolicProbe:
         self.rowptr = np.zeros(net.nB + 1, np.int32)
         self.colind = np.zeros(ynnz.value, np.int32)
         lib().pf_probe_get(h, _p(self.perm), _p(self.slot_e), _p(self.rowptr), _p(self.colind), None)
+        self.lvl_ptr = np.zeros(self.levels + 1, np.int32)
+        self.colptr = np.zeros(self.n + 1, np.int32)
+        lib().pf_probe_levels(h, _p(self.lvl_ptr), _p(self.colptr))
 
     def factor_solve(self, blocks: np.ndarray, rhs: np.ndarray) -> np.ndarray:
         b = np.ascontiguousarray(blocks, np.float64).copy()

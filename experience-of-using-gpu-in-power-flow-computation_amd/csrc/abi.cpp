@@ -136,6 +136,14 @@ pf_status pf_probe_get(const pf_symbolic_probe *p, int *perm, int *slot_e, int *
     return PF_OK;
 }
 
+// level structure: lvl_ptr (levels+1), colptr (n+1) -- any may be NULL
+pf_status pf_probe_levels(const pf_symbolic_probe *p, int *lvl_ptr, int *colptr) {
+    if (!p) return PF_INVALID_ARG;
+    if (lvl_ptr) std::memcpy(lvl_ptr, p->sym.lvl_ptr.data(), sizeof(int) * p->sym.lvl_ptr.size());
+    if (colptr) std::memcpy(colptr, p->sym.colptr.data(), sizeof(int) * p->sym.colptr.size());
+    return PF_OK;
+}
+
 // blocks: nslot*4 doubles (in), x: 2n doubles in node order (in: rhs, out: solution)
 pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x) {
     if (!p || !blocks || !x) return PF_INVALID_ARG;

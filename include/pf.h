@@ -181,6 +181,9 @@ void pf_probe_free(pf_symbolic_probe *p);
  * Ybus entry whose Jacobian block lands in slot s (-1: fill-in)             */
 pf_status pf_probe_get(const pf_symbolic_probe *p, int *perm, int *slot_e, int *rowptr,
                        int *colind, int *ynnz);
+/* elimination-tree levels: lvl_ptr [levels+1] (nodes of level l are the
+ * positions lvl_ptr[l] .. lvl_ptr[l+1]-1), column pointers of L [n+1]      */
+pf_status pf_probe_levels(const pf_symbolic_probe *p, int *lvl_ptr, int *colptr);
 /* blocks: nslot 2x2 blocks (row-major), x: 2n right-hand side in node order */
 pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x);
 

@@ -8,9 +8,14 @@ kernels.  value = converged solves / second over the timed steps.
 
 Printed beside it, in the same JSON line:
   * plain FP64 GPU NR on the same 118-bus scenarios, one scenario at a time
-    (eager launches, one host sync per iteration) and the batched/plain ratio;
-  * the single 15,104-bus solve: plain FP64 eager vs FP32-factor + FP64
-    refinement in one captured graph per iteration;
+    (eager launches, one host sync per iteration, the sparse LU one
+    elimination-tree level at a time) and the batched/plain ratio;
+  * the single 15,104-bus solve: that plain FP64 path vs the optimised one
+    (dense separator block, one captured graph per iteration) in FP64 and
+    with the FP32 factor + FP64 refinement;
+  * the learned warm start (944-bus network and the headline 118-bus
+    scenarios): training on the solver's own answers, iterations and
+    solves/s against the flat start;
   * roofline of the dominant persistent kernel from its HIP-event time;
   * the oracle (FP64 C restatement of the reference CPU path) timed on the
     host cores over the same workload, and the accuracy of the GPU answers
@@ -68,6 +73,92 @@ def bytes_per_iter(nB, nnzY_off, n, nnzL, nPV, s_lu, refine):
     return per, topo
 
 
+def log(*a):
+    """Progress on stderr (the JSON line alone goes to stdout)."""
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE of the same workload, each scaled by
+    what the counter reads for a plain device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "k_batch_pmc_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 4 and parts[0] == "TRAFFIC" and " ".join(parts[1:3]) == kernel:
+                    return int(parts[3]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
+def warm_start_leg(pf, synth, net, opts, n_train, n_eval, epochs, seed_train, seed_eval):
+    """Learned warm start on one topology: solve n_train perturbed scenarios
+    with the batch solver (flat start), train the MLP (bus P, Q and the PV
+    setpoints -> |V|, theta) on the converged answers with the library's own
+    kernels, then solve n_eval held-out scenarios from the flat start and from
+    the predicted start (flat-start fallback inside the kernel) and compare
+    iterations and solves/s.  The warm figure includes the inference time."""
+    is_pv = np.zeros(net.nB, bool)
+    is_pv[net.pv] = True
+    vset = np.where(is_pv, net.V, 0.0)
+    vset[net.slack] = net.V[net.slack]
+
+    def feats(P, Q):
+        return np.concatenate([P, Q, np.repeat(vset[None, :], len(P), 0)], 1).astype(np.float32)
+
+    def solve(P, Q, L, V0=None, T0=None, reps=1):
+        b = pf.Batch([net], opts)
+        b.set_scenarios(np.zeros(len(P), np.int32), P.ravel(), Q.ravel(), L.ravel(),
+                        None if V0 is None else V0.ravel(), None if T0 is None else T0.ravel())
+        b.solve()
+        ms = min(b.solve() for _ in range(reps))
+        V, th, it, cv, mm = b.results()
+        return V.reshape(len(P), -1), th.reshape(len(P), -1), it, cv, ms
+
+    t = time.perf_counter()
+    log(f"warm start {net.nB}-bus: solving {n_train} training scenarios")
+    P, Q, L = synth.scenario_batch(net.Pg, net.Pload, net.Qg, net.Qload, is_pv,
+                                   np.arange(seed_train, seed_train + n_train))[:3]
+    V, th, it, cv, ms_data = solve(P, Q, L)
+    log(f"warm start {net.nB}-bus: training, {epochs} epochs")
+    ok = cv.astype(bool)
+    X, Y = feats(P, Q)[ok], np.concatenate([V, th], 1).astype(np.float32)[ok]
+    t_data = time.perf_counter() - t
+    w = pf.WarmStart(3 * net.nB, 2 * net.nB, 512, 3, seed=11)
+    w.set_norm(X.mean(0), X.std(0) + 1e-6, Y.mean(0), Y.std(0) + 1e-9)
+    t = time.perf_counter()
+    losses = w.fit(X, Y, epochs=epochs, batch=256, lr=1e-3, weight_decay=1e-5)
+    t_fit = time.perf_counter() - t
+    log(f"warm start {net.nB}-bus: evaluating {n_eval} scenarios")
+    Pt, Qt, Lt = synth.scenario_batch(net.Pg, net.Pload, net.Qg, net.Qload, is_pv,
+                                      np.arange(seed_eval, seed_eval + n_eval))[:3]
+    Xt = feats(Pt, Qt)
+    w.predict(Xt[:256])
+    t = time.perf_counter()
+    Yt = w.predict(Xt)
+    t_pred = time.perf_counter() - t
+    Vf, thf, itf, cvf, msf = solve(Pt, Qt, Lt, reps=3)
+    Vw, thw, itw, cvw, msw = solve(Pt, Qt, Lt, Yt[:, :net.nB].astype(np.float64),
+                                   Yt[:, net.nB:].astype(np.float64), reps=3)
+    both = cvf.astype(bool) & cvw.astype(bool)
+    return {"buses": net.nB, "train_scenarios": int(n_train), "train_converged": int(cv.sum()),
+            "eval_scenarios": int(n_eval), "epochs": epochs,
+            "mlp": f"{3 * net.nB}->512x3 GELU->{2 * net.nB}, bf16 weights, fp32 accumulate",
+            "train_data_s": round(t_data, 2), "fit_s": round(t_fit, 2),
+            "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 6)],
+            "iterations_flat_mean": round(float(itf.mean()), 3),
+            "iterations_warm_mean": round(float(itw.mean()), 3),
+            "converged_flat": int(cvf.sum()), "converged_warm": int(cvw.sum()),
+            "solve_ms_flat": round(msf, 3), "solve_ms_warm": round(msw, 3),
+            "predict_ms": round(1e3 * t_pred, 3),
+            "solves_per_s_flat": round(float(cvf.sum()) / (msf * 1e-3), 1),
+            "solves_per_s_warm_incl_predict": round(float(cvw.sum()) / ((msw + 1e3 * t_pred) * 1e-3), 1),
+            "max_abs_dV_warm_vs_flat_pu": float(np.abs(Vw[both] - Vf[both]).max()) if both.any() else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,7 +170,28 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single 15,104-bus solve")
     ap.add_argument("--refine", type=int, default=2)
+    ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start legs")
+    ap.add_argument("--warm-train", type=int, default=50000)
+    ap.add_argument("--warm-eval", type=int, default=4096)
+    ap.add_argument("--warm-epochs", type=int, default=8)
+    ap.add_argument("--tune-cache", default=os.path.expanduser("~/.cache/libpf/batch_shapes.txt"),
+                    help="launch-shape cache of the batch autotuner ('' = measure, keep nothing)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 64 << 20
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                          "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is not provided",
                           "n_gpus": args.gpus}))
@@ -100,8 +212,14 @@ def main():
     L = np.concatenate([B[2].ravel(), A[2].ravel()])
     opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, precision=pf.MIXED,
                       refine_steps=args.refine)
+    cache = args.tune_cache or None
+    if cache:
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
     batch = pf.Batch([n118, n1888], opts)
     batch.set_scenarios(net_of, P, Q, L)
+    t = time.perf_counter()
+    n_tuned = batch.autotune(cache, reps=1)
+    tune_s = time.perf_counter() - t
     for _ in range(args.warmup):
         batch.solve()
     t0 = time.perf_counter()
@@ -118,12 +236,15 @@ def main():
     extras = {}
     # roofline of each persistent kernel from its own timed launch
     roof = None
+    log("headline batch done", f"{value:.1f} solves/s")
     if not args.no_extras:
+        log("per-kernel launches and plain FP64 GPU NR")
         kern = {}
-        for name, net, S, k in (("k_batch<float,lds> 118-bus", n118, A, args.n118),
-                                ("k_batch<float,global> 1888-bus", n1888, B, args.n1888)):
+        for name, net, S, k in (("k_batch 118-bus", n118, A, args.n118),
+                                ("k_batch 1888-bus", n1888, B, args.n1888)):
             b1 = pf.Batch([net], opts)
             b1.set_scenarios(np.zeros(k, np.int32), S[0].ravel(), S[1].ravel(), S[2].ravel())
+            b1.autotune(cache, reps=1)
             b1.solve()
             ts = [b1.solve() for _ in range(max(3, args.steps // 2))]
             _, _, it1, cv1, _ = b1.results()
@@ -131,22 +252,24 @@ def main():
             rp = pr.rowptr
             coorow = np.repeat(np.arange(net.nB), np.diff(rp))
             off = int(np.sum((coorow != net.slack) & (pr.colind != net.slack) & (coorow != pr.colind)))
-            per, topo = bytes_per_iter(net.nB, off, pr.n, pr.nnzL, net.nPV, 4, 2)
+            per, topo = bytes_per_iter(net.nB, off, pr.n, pr.nnzL, net.nPV, 4, args.refine)
             alg = per * int(it1.sum()) + topo * int(it1.max())
             kern[name] = dict(ms=float(np.mean(ts)), bytes=alg, iters=int(it1.sum()),
-                              conv=int(cv1.sum()), n=k)
+                              conv=int(cv1.sum()), n=k, shape=b1.shape(0))
         dom = max(kern, key=lambda k: kern[k]["ms"])
         d = kern[dom]
         ach = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+        traffic, tsrc = measured_traffic(dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
-                "launch_ms": round(d["ms"], 4), "alg_bytes_per_launch": d["bytes"]}
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": tsrc, "launch_ms": round(d["ms"], 4), "alg_bytes_per_launch": d["bytes"],
+                "shape": d["shape"]}
         extras["kernels"] = {k: {"launch_ms": round(v["ms"], 4), "alg_bytes": v["bytes"],
                                  "scenario_iterations": v["iters"], "converged": v["conv"],
-                                 "scenarios": v["n"]} for k, v in kern.items()}
+                                 "scenarios": v["n"], "shape": v["shape"]} for k, v in kern.items()}
 
         # plain FP64 GPU NR, one 118-bus scenario at a time (eager, host sync per iteration)
-        sv = pf.Solver(n118, pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1))
+        sv = pf.Solver(n118, pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, lu_blocked=0))
         m = min(64, args.n118)
         sv.set_injections(A[0][0], A[1][0], A[2][0])
         sv.reset()
@@ -158,21 +281,22 @@ def main():
             sv.reset()
             pconv += sv.solve()["converged"]
         plain = pconv / (time.perf_counter() - t)
-        b118 = kern["k_batch<float,lds> 118-bus"]
+        b118 = kern["k_batch 118-bus"]
         batched118 = b118["conv"] / (b118["ms"] * 1e-3)
         extras["plain_fp64_gpu_118"] = {"solves_per_s": round(plain, 1), "sample": m}
         extras["batched_mixed_gpu_118"] = {"solves_per_s": round(batched118, 1)}
         extras["batched_vs_plain_118"] = round(batched118 / plain, 1)
 
     if not args.no_extras and not args.no_single:
+        log("single 15,104-bus solve")
         # single 15,104-bus solve
         p15 = os.path.join(tmp, "c15104.cdfx")
         synth.write_cdf(synth.case15104(), p15)
         n15 = pf.Network(p15)
         res = {}
-        for tag, kw in (("plain_fp64_eager", dict()),
+        for tag, kw in (("plain_fp64_eager", dict(lu_blocked=0)),
                         ("fp64_graph", dict(use_graph=1)),
-                        ("mixed_graph", dict(use_graph=1, precision=pf.MIXED))):
+                        ("mixed_graph", dict(use_graph=1, precision=pf.MIXED, refine_steps=args.refine))):
             s15 = pf.Solver(n15, pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, **kw))
             s15.solve()
             times = []
@@ -187,8 +311,24 @@ def main():
         res["lu"] = lu
         extras["single_15104"] = res
 
+    if not args.no_extras and not args.no_warm:
+        # learned warm start: the 944-bus 8-copy network (50k training, 4096
+        # held-out scenarios) and the 118-bus network of the headline batch
+        p944 = os.path.join(tmp, "c944.cdf")
+        synth.write_cdf(synth.case944(), p944)
+        n944 = pf.Network(p944)
+        extras["warm_start_944"] = warm_start_leg(pf, synth, n944, opts, args.warm_train, args.warm_eval,
+                                                  args.warm_epochs, 4000000, 5000000)
+        ws118 = warm_start_leg(pf, synth, n118, opts, args.warm_train, args.n118,
+                               args.warm_epochs, 6000000, 3000000)
+        extras["warm_start_118"] = ws118
+        if "plain_fp64_gpu_118" in extras:
+            extras["batched_warm_vs_plain_118"] = round(
+                ws118["solves_per_s_warm_incl_predict"] / extras["plain_fp64_gpu_118"]["solves_per_s"], 1)
+
     cpu = None
     if not args.no_cpu:
+        log("oracle on the host cores")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         O.build()
@@ -232,6 +372,9 @@ def main():
            "config": {"workload": f"{args.n118} x 118-bus + {args.n1888} x 1888-bus scenarios in one "
                                   f"persistent launch, Q limits on, tol 1e-8, flat start",
                       "scenarios": nsc, "converged_per_step": n_conv,
+                      "launch_shapes": {"118-bus": batch.shape(0), "1888-bus": batch.shape(1),
+                                        "autotune_candidates_measured": n_tuned,
+                                        "autotune_s": round(tune_s, 2)},
                       "event_ms_per_step": round(float(np.mean(ev_ms)), 4)},
            "roofline": roof, "cpu_baseline": cpu}
     out.update(extras)

@@ -36,7 +36,7 @@ class Options(C.Structure):
                 ("arq_entrada", C.c_char * 1024), ("verbose_mode", C.c_int),
                 ("laconic_mode", C.c_int), ("output_bench", C.c_int), ("output_ans", C.c_int),
                 ("output_iter", C.c_int), ("streams", C.c_int), ("precision", C.c_int),
-                ("refine_steps", C.c_int), ("use_graph", C.c_int)]
+                ("refine_steps", C.c_int), ("use_graph", C.c_int), ("lu_blocked", C.c_int)]
 
 
 class NetworkInfo(C.Structure):
@@ -100,6 +100,8 @@ def lib():
         L.pf_probe_get.argtypes = [P, P, P, P, P, P]
         L.pf_probe_factor_solve.argtypes = [P, P, P]
         L.pf_probe_levels.argtypes = [P, P, P]
+        L.pf_probe_chain.argtypes = [P, P, P]
+        L.pf_probe_parts.argtypes = [P, P, P, P, P, P]
         if hasattr(L, "pf_warmstart_create"):
             L.pf_warmstart_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int,
                                               C.POINTER(P)]
@@ -119,6 +121,9 @@ def lib():
             L.pf_batch_set_scenarios.argtypes = [P, C.c_int, P, P, P, P, P, P]
             L.pf_batch_solve.argtypes = [P, P]
             L.pf_batch_get.argtypes = [P, P, P, P, P, P]
+            L.pf_batch_autotune.argtypes = [P, C.c_char_p, C.c_int, P]
+            L.pf_batch_get_shape.argtypes = [P, C.c_int, P, P, P, P]
+            L.pf_batch_set_shape.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -205,6 +210,24 @@ class SymbolicProbe:
         self.lvl_ptr = np.zeros(self.levels + 1, np.int32)
         self.colptr = np.zeros(self.n + 1, np.int32)
         lib().pf_probe_levels(h, _p(self.lvl_ptr), _p(self.colptr))
+
+    def parts(self):
+        """(part_ptr reshaped per group: [n_parts, part_levels + 1] boundaries,
+        part_nodes, parent) of the subtree groups below the narrow top."""
+        npart, nl = C.c_int(), C.c_int()
+        _chk(lib().pf_probe_parts(self.h, C.byref(npart), C.byref(nl), None, None, None), "pf_probe_parts")
+        ptr = np.zeros(npart.value * nl.value + 1, np.int32)
+        nodes = np.zeros(max(int(self.lvl_ptr[nl.value]), 1), np.int32)
+        parent = np.zeros(self.n, np.int32)
+        _chk(lib().pf_probe_parts(self.h, None, None, _p(ptr), _p(nodes), _p(parent)), "pf_probe_parts")
+        return npart.value, nl.value, ptr, nodes[:int(self.lvl_ptr[nl.value])], parent
+
+    @property
+    def chain(self):
+        """(length, first node) of the trailing dense chain factored as one block."""
+        m, c0 = C.c_int(), C.c_int()
+        _chk(lib().pf_probe_chain(self.h, C.byref(m), C.byref(c0)), "pf_probe_chain")
+        return m.value, c0.value
 
     def factor_solve(self, blocks: np.ndarray, rhs: np.ndarray) -> np.ndarray:
         b = np.ascontiguousarray(blocks, np.float64).copy()
@@ -363,6 +386,25 @@ class Batch:
         ms = C.c_double()
         _chk(lib().pf_batch_solve(self.h, C.byref(ms)), "pf_batch_solve")
         return ms.value
+
+    def autotune(self, cache_path=None, reps: int = 2) -> int:
+        """Pick the fastest launch shape per network for the current
+        scenarios (or apply a cached one); returns the number of candidate
+        shapes measured."""
+        n = C.c_int()
+        _chk(lib().pf_batch_autotune(self.h, None if cache_path is None else os.fsencode(cache_path),
+                                     reps, C.byref(n)), "pf_batch_autotune")
+        return n.value
+
+    def set_shape(self, net: int, mode: int, block: int, per_cu: int = 0):
+        _chk(lib().pf_batch_set_shape(self.h, net, mode, block, per_cu), "pf_batch_set_shape")
+
+    def shape(self, net: int) -> dict:
+        m, b, g = C.c_int(), C.c_int(), C.c_int()
+        lds = C.c_longlong()
+        _chk(lib().pf_batch_get_shape(self.h, net, C.byref(m), C.byref(b), C.byref(g), C.byref(lds)),
+             "pf_batch_get_shape")
+        return {"mode": m.value, "block": b.value, "grid": g.value, "lds_bytes": lds.value}
 
     def results(self):
         V = np.zeros(self.total)

@@ -113,7 +113,9 @@ pf_status pf_probe_create(const pf_network *net, pf_symbolic_probe **out, int *n
     if (!net || !out || net->nB < 2) return PF_INVALID_ARG;
     auto p = std::make_unique<pf_symbolic_probe>();
     p->yp = pf::build_ypattern(*net);
-    p->sym = pf::analyze(*net, p->yp, 16);
+    p->sym = pf::analyze(*net, p->yp, 16, pf::kChainMax);   // the single-network solver's analysis
+    if (!p->sym.level_ordered()) return PF_INVALID_ARG;
+    pf::partition_subtrees(p->sym, 256);
     if (n) *n = p->sym.n;
     if (nnzL) *nnzL = p->sym.nnzL;
     if (nslot) *nslot = p->sym.nslot();
@@ -144,11 +146,38 @@ pf_status pf_probe_levels(const pf_symbolic_probe *p, int *lvl_ptr, int *colptr)
     return PF_OK;
 }
 
+// trailing dense chain factored as one block: length and first node
+pf_status pf_probe_chain(const pf_symbolic_probe *p, int *m, int *c0) {
+    if (!p) return PF_INVALID_ARG;
+    if (m) *m = p->sym.ch_m;
+    if (c0) *c0 = p->sym.ch_c0;
+    return PF_OK;
+}
+
+// subtree groups below the tail (as the solver builds them for 256 CUs) and
+// the elimination tree; any pointer may be NULL
+pf_status pf_probe_parts(const pf_symbolic_probe *p, int *n_parts, int *part_levels, int *part_ptr,
+                         int *part_nodes, int *parent) {
+    if (!p) return PF_INVALID_ARG;
+    const pf::Symbolic &s = p->sym;
+    if (n_parts) *n_parts = s.n_parts;
+    if (part_levels) *part_levels = s.tail_start;
+    if (part_ptr) std::memcpy(part_ptr, s.part_ptr.data(), sizeof(int) * s.part_ptr.size());
+    if (part_nodes) std::memcpy(part_nodes, s.part_nodes.data(), sizeof(int) * s.part_nodes.size());
+    if (parent) std::memcpy(parent, s.parent.data(), sizeof(int) * s.n);
+    return PF_OK;
+}
+
 // blocks: nslot*4 doubles (in), x: 2n doubles in node order (in: rhs, out: solution)
 pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x) {
     if (!p || !blocks || !x) return PF_INVALID_ARG;
-    if (!pf::host_block_factor(p->sym, blocks)) return PF_SINGULAR;
-    pf::host_block_solve(p->sym, blocks, x);
+    // factor in a copy with room for the dense chain block, then hand the
+    // slots back (the chain's S^-1 stays internal)
+    std::vector<double> b(4 * (size_t)p->sym.nblk(), 0.0);
+    std::memcpy(b.data(), blocks, sizeof(double) * 4 * (size_t)p->sym.nslot());
+    if (!pf::host_block_factor(p->sym, b.data())) return PF_SINGULAR;
+    pf::host_block_solve(p->sym, b.data(), x);
+    std::memcpy(blocks, b.data(), sizeof(double) * 4 * (size_t)p->sym.nslot());
     return PF_OK;
 }
 

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -36,7 +37,8 @@ struct BatchTopo {
     DevNet N;
     DevSym Y;
     const double *V0flat, *th0flat;     // flat start of this topology
-    int nslot;
+    int nslot;                          // Jacobian slots
+    int nblk;                           // factor blocks: slots, row-ordered L, chain inverse
 };
 
 struct BatchJob {
@@ -56,6 +58,13 @@ struct BatchJob {
 };
 
 constexpr int GL = 16;   // lanes per node in the in-workgroup LU and solves
+// trailing dense chain: at most kBatchChain nodes (inverted where the
+// working set lives: the registers are taken by the rest of the solve)
+constexpr int kBatchChain = 32;
+// levels of at least kWideLevel nodes: a lane group per node; narrower ones:
+// the whole workgroup on the level's destinations.  A constant, not the
+// number of lane groups, so that every workgroup size sums in the same order.
+constexpr int kWideLevel = 32;
 
 // per-scenario working set, in LDS or in a global slice
 template <typename T>
@@ -66,20 +75,21 @@ struct Work {
     T *blk;
 };
 
+// nblk: factor blocks (Jacobian slots and the row-ordered copy of L)
 template <typename T>
-__host__ __device__ inline long long work_bytes(int nB, int n, int nPV, int nslot) {
+__host__ __device__ inline long long work_bytes(int nB, int n, int nPV, int nblk) {
     long long b = 0;
     b += 8LL * nB * 5;                 // V th P Q ql
     b += 8LL * 2 * n * 3;              // g x r
     b += 16LL * nB * 2;                // E I
     b = (b + 15) & ~15LL;
-    b += (long long)sizeof(T) * 4 * nslot;
+    b += (long long)sizeof(T) * 4 * nblk;
     b += nB + nPV + 16;
     return (b + 255) & ~255LL;
 }
 
 template <typename T>
-__device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nslot) {
+__device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nblk) {
     Work<T> w;
     char *p = base;
     auto take = [&](size_t bytes) { char *q = p; p += bytes; return q; };
@@ -94,7 +104,7 @@ __device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nslot) {
     w.E = (double2 *)take(16 * (size_t)nB);
     w.I = (double2 *)take(16 * (size_t)nB);
     p = base + ((p - base + 15) & ~(size_t)15);
-    w.blk = (T *)take(sizeof(T) * 4 * (size_t)nslot);
+    w.blk = (T *)take(sizeof(T) * 4 * (size_t)nblk);
     w.isq = (unsigned char *)take(nB);
     w.limQ = (unsigned char *)take(nPV);
     return w;
@@ -143,22 +153,56 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
             dinv.d = __shfl(di.d, 0, GL);
         }
         if (t < nd && t > 0) {
-            store_b4(blk, L0 + p, mul(a, dinv));
+            const B4<T> l = mul(a, dinv);
+            store_b4(blk, L0 + p, l);
+            store_b4(blk, U0 + Y.nnzL + Y.lr_pos[p], l);
             store_b4(blk, U0 + p, u);
         }
     }
 }
 
+// Forward and backward sweeps, GL lanes per node.  The first entry of the
+// first node a lane group takes on a level (index and factor block, which do
+// not depend on x) is loaded one level ahead, so its latency overlaps the
+// previous level's arithmetic and barrier; the summation order is the same
+// as without it.
 template <typename T>
 __device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
     const int grp = (int)threadIdx.x / GL, lane = (int)threadIdx.x % GL, ng = (int)blockDim.x / GL;
-    for (int l = 0; l < Y.nlev; l++) {
+    const int LR0 = Y.n + 2 * Y.nnzL, U0 = Y.n + Y.nnzL;
+    const int nl = Y.nlev_s;
+    auto fetch_fwd = [&](int l, int &k, B4<T> &L) {
+        k = -1;
+        const int s = Y.lvl_ptr[l];
+        if (grp < Y.lvl_ptr[l + 1] - s) {
+            const int j = s + grp, q = Y.rs_ptr[j] + lane;
+            if (q < Y.rs_ptr[j + 1]) {
+                k = Y.rs_k[q];
+                L = load_b4(blk, LR0 + q);
+            }
+        }
+    };
+    int pk = -1;
+    B4<T> pL{0, 0, 0, 0};
+    if (nl > 0) fetch_fwd(0, pk, pL);
+    for (int l = 0; l < nl; l++) {
+        int nk = -1;
+        B4<T> nL{0, 0, 0, 0};
+        if (l + 1 < nl) fetch_fwd(l + 1, nk, nL);
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
         for (int i = grp; i < c; i += ng) {
-            const int j = Y.lvl_nodes[s + i];
+            const int j = (s + i);
             double a0 = 0, a1 = 0;
-            for (int q = Y.rs_ptr[j] + lane; q < Y.rs_ptr[j + 1]; q += GL) {
-                B4<T> L = load_b4(blk, Y.n + Y.rs_p[q]);
+            int q = Y.rs_ptr[j] + lane;
+            if (i == grp) {
+                if (pk >= 0) {
+                    a0 += (double)pL.a * x[2 * pk] + (double)pL.b * x[2 * pk + 1];
+                    a1 += (double)pL.c * x[2 * pk] + (double)pL.d * x[2 * pk + 1];
+                }
+                q += GL;
+            }
+            for (; q < Y.rs_ptr[j + 1]; q += GL) {
+                B4<T> L = load_b4(blk, LR0 + q);
                 const int k = Y.rs_k[q];
                 a0 += (double)L.a * x[2 * k] + (double)L.b * x[2 * k + 1];
                 a1 += (double)L.c * x[2 * k] + (double)L.d * x[2 * k + 1];
@@ -174,14 +218,42 @@ __device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
             }
         }
         __syncthreads();
+        pk = nk;
+        pL = nL;
     }
-    const int U0 = Y.n + Y.nnzL;
-    for (int l = Y.nlev - 1; l >= 0; l--) {
+    auto fetch_bwd = [&](int l, int &r, B4<T> &U, B4<T> &D) {
+        r = -1;
+        const int s = Y.lvl_ptr[l];
+        if (grp < Y.lvl_ptr[l + 1] - s) {
+            const int j = s + grp, p = Y.colptr[j] + lane;
+            if (lane == 0) D = load_b4(blk, j);
+            if (p < Y.colptr[j + 1]) {
+                r = Y.rowind[p];
+                U = load_b4(blk, U0 + p);
+            }
+        }
+    };
+    int pr = -1;
+    B4<T> pU{0, 0, 0, 0}, pD{0, 0, 0, 0};
+    if (nl > 0) fetch_bwd(nl - 1, pr, pU, pD);
+    if (Y.ch_m > 0) chain_solve<T, GL, kBatchChain>(Y, blk, x);
+    for (int l = nl - 1; l >= 0; l--) {
+        int nr = -1;
+        B4<T> nU{0, 0, 0, 0}, nD{0, 0, 0, 0};
+        if (l > 0) fetch_bwd(l - 1, nr, nU, nD);
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
         for (int i = grp; i < c; i += ng) {
-            const int j = Y.lvl_nodes[s + i];
+            const int j = (s + i);
             double a0 = 0, a1 = 0;
-            for (int p = Y.colptr[j] + lane; p < Y.colptr[j + 1]; p += GL) {
+            int p = Y.colptr[j] + lane;
+            if (i == grp) {
+                if (pr >= 0) {
+                    a0 += (double)pU.a * x[2 * pr] + (double)pU.b * x[2 * pr + 1];
+                    a1 += (double)pU.c * x[2 * pr] + (double)pU.d * x[2 * pr + 1];
+                }
+                p += GL;
+            }
+            for (; p < Y.colptr[j + 1]; p += GL) {
                 B4<T> U = load_b4(blk, U0 + p);
                 const int r = Y.rowind[p];
                 a0 += (double)U.a * x[2 * r] + (double)U.b * x[2 * r + 1];
@@ -194,12 +266,15 @@ __device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
             }
             if (lane == 0) {
                 const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
-                B4<T> D = load_b4(blk, j);
+                const B4<T> D = (i == grp) ? pD : load_b4(blk, j);
                 x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
                 x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
             }
         }
         __syncthreads();
+        pr = nr;
+        pU = nU;
+        pD = nD;
     }
 }
 
@@ -432,13 +507,17 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
         // workgroup on the level's destinations (coop_factor_level)
         {
             const int grp = tid / GL, lane = tid % GL, ng = (int)blockDim.x / GL;
-            for (int l = 0; l < Y.nlev; l++) {
+            for (int l = 0; l < Y.nlev_s; l++) {
                 const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
-                if (c >= ng) {
-                    for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, Y.lvl_nodes[s + i], lane, &s_sing);
+                if (c >= kWideLevel) {
+                    for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, (s + i), lane, &s_sing);
                 } else {
                     coop_factor_level<T, GL>(Y, w.blk, l, &s_sing);
                 }
+                __syncthreads();
+            }
+            if (Y.ch_m > 0) {
+                chain_factor_mem<T, GL, kBatchChain>(Y, w.blk, &s_sing);
                 __syncthreads();
             }
         }
@@ -486,14 +565,48 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
     __syncthreads();
 }
 
-template <typename T, bool LDS>
-__global__ void __launch_bounds__(1024) k_batch(BatchTopo tp, BatchJob J) {
+// bytes of LDS that MODE 1 uses: the step vectors g, x, r and the level /
+// row-structure metadata the triangular solves and the factor walk
+template <typename T>
+__host__ __device__ inline long long partial_lds_bytes(int n, int nlev) {
+    long long b = 16LL * n * 3;                              // g x r
+    b += 4LL * ((nlev + 1) * 2 + 3 * (n + 1));               // lvl_ptr ld_ptr rs_ptr colptr fd_ptr
+    return (b + 255) & ~255LL;
+}
+
+// MODE 2: the whole per-scenario working set in LDS.
+// MODE 1: working set in a global slice, except the step vectors and the
+//         level / row-structure metadata, staged in LDS (latency of the
+//         level-by-level walk); MODE 0: everything in the global slice.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(1024) k_batch(BatchTopo tp0, BatchJob J) {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_item, s_it, s_conv;
     __shared__ double s_maxg;
+    BatchTopo tp = tp0;
     const DevNet &N = tp.N;
-    char *base = LDS ? smem : J.gws + (long long)blockIdx.x * J.ws_bytes;
-    Work<T> w = carve<T>(base, N.nB, tp.Y.n, N.nPV, tp.nslot);
+    char *base = (MODE == 2) ? smem : J.gws + (long long)blockIdx.x * J.ws_bytes;
+    Work<T> w = carve<T>(base, N.nB, tp.Y.n, N.nPV, tp.nblk);
+    if (MODE == 1) {
+        const int n = tp.Y.n, nlev = tp.Y.nlev;
+        double *v = reinterpret_cast<double *>(smem);
+        w.g = v;
+        w.x = v + 2 * n;
+        w.r = v + 4 * n;
+        int *ip = reinterpret_cast<int *>(v + 6 * n);
+        auto stage = [&](const int *src, int cnt) {
+            int *dst = ip;
+            for (int i = threadIdx.x; i < cnt; i += blockDim.x) dst[i] = src[i];
+            ip += cnt;
+            return (const int *)dst;
+        };
+        tp.Y.lvl_ptr = stage(tp0.Y.lvl_ptr, nlev + 1);
+        tp.Y.ld_ptr = stage(tp0.Y.ld_ptr, nlev + 1);
+        tp.Y.rs_ptr = stage(tp0.Y.rs_ptr, n + 1);
+        tp.Y.colptr = stage(tp0.Y.colptr, n + 1);
+        tp.Y.fd_ptr = stage(tp0.Y.fd_ptr, n + 1);
+        __syncthreads();
+    }
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(J.counter, 1);
         __syncthreads();
@@ -556,14 +669,19 @@ struct BatchNet {
     YPattern yp;
     Symbolic sym;
     BatchTopo tp{};
-    bool lds = false;
+    bool lds = false;          // the whole working set fits in LDS (MODE 2)
+    long long partial = 0;     // MODE 1 LDS bytes, 0 if it does not fit
+    int mode = 0;              // k_batch MODE
+    long long shm = 0;         // dynamic LDS bytes
     int block = 256;
     int grid = 0;
+    int per_cu = 1;            // workgroups per CU the grid is sized for
     long long ws = 0;
     std::vector<int> scen;
     // device
     int *d_scen = nullptr;
     char *d_ws = nullptr;
+    size_t ws_cap = 0;         // bytes allocated at d_ws
 };
 
 struct pf_batch {
@@ -610,14 +728,30 @@ struct pf_batch {
             if (s) (void)hipStreamDestroy(s);
     }
     pf_status add_net(const pf_network &net);
+    pf_status launch_net(size_t i, hipStream_t s);
     pf_status launch_all();
+    pf_status configure(BatchNet &nb, int mode, int block, int per_cu);
+    pf_status occupancy(const BatchNet &nb, int mode, int block, int *per_cu);
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
 };
 
 pf_status pf_batch::add_net(const pf_network &src) {
     auto b = std::make_unique<BatchNet>();
+    int max_lds_i = 0;
+    BHIP(hipDeviceGetAttribute(&max_lds_i, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    const long long max_lds = max_lds_i - 8192;   // room for the kernel's static LDS
     b->net = src;
     b->yp = build_ypattern(src);
-    b->sym = analyze(src, b->yp, 0);
+    b->sym = analyze(src, b->yp, 0, opt.lu_blocked ? kBatchChain : 0);
+    if (!b->sym.level_ordered()) {
+        g_batch_error = "symbolic analysis: nodes not numbered level by level";
+        return PF_INVALID_ARG;
+    }
     const pf_network &net = b->net;
     const YPattern &yp = b->yp;
     const Symbolic &sym = b->sym;
@@ -651,6 +785,8 @@ pf_status pf_batch::add_net(const pf_network &src) {
     BCHK(upload((double **)&N.qmax, net.Qmax));
     BCHK(upload((double **)&N.vset, net.Vset));
     Y.n = n; Y.nnzL = sym.nnzL; Y.nslot = sym.nslot(); Y.nlev = sym.nlev;
+    Y.ch_m = sym.ch_m; Y.ch_c0 = sym.ch_c0; Y.nlev_s = sym.nlev_s;
+    BCHK(upload((int **)&Y.ch_split, sym.ch_split));
     BCHK(upload((int **)&Y.colptr, sym.colptr));
     BCHK(upload((int **)&Y.rowind, sym.rowind));
     Y.pcol = nullptr;
@@ -663,6 +799,7 @@ pf_status pf_batch::add_net(const pf_network &src) {
     BCHK(upload((int **)&Y.rs_ptr, sym.rs_ptr));
     BCHK(upload((int **)&Y.rs_k, sym.rs_k));
     BCHK(upload((int **)&Y.rs_p, sym.rs_p));
+    BCHK(upload((int **)&Y.lr_pos, sym.lr_pos));
     BCHK(upload((int **)&Y.slot_e, sym.slot_e));
     BCHK(upload((int **)&Y.ld_ptr, sym.ld_ptr));
     BCHK(upload((int **)&Y.ld_node, sym.ld_node));
@@ -670,28 +807,93 @@ pf_status pf_batch::add_net(const pf_network &src) {
     BCHK(upload((double **)&b->tp.V0flat, net.V));
     BCHK(upload((double **)&b->tp.th0flat, net.theta));
     b->tp.nslot = sym.nslot();
+    b->tp.nblk = sym.nblk();
     k_ybus<<<(yp.nnz + 255) / 256, 256, 0, st[0]>>>(N, yp.nnz);
     BHIP(hipGetLastError());
     BHIP(hipStreamSynchronize(st[0]));
     const bool mixed = opt.precision != PF_FP64;
-    b->ws = mixed ? work_bytes<float>(nB, n, net.nPV, sym.nslot())
-                  : work_bytes<double>(nB, n, net.nPV, sym.nslot());
+    b->ws = mixed ? work_bytes<float>(nB, n, net.nPV, sym.nblk())
+                  : work_bytes<double>(nB, n, net.nPV, sym.nblk());
     b->lds = b->ws <= kLdsLimit;
     b->block = b->lds ? 256 : 1024;
+    if (b->lds) {
+        b->mode = 2;
+        b->shm = b->ws;
+    } else {
+        long long pb = mixed ? partial_lds_bytes<float>(n, sym.nlev) : partial_lds_bytes<double>(n, sym.nlev);
+        hipError_t e = hipSuccess;
+        if (pb <= max_lds) {
+            if (mixed) e = hipFuncSetAttribute((const void *)k_batch<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pb);
+            else e = hipFuncSetAttribute((const void *)k_batch<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pb);
+        }
+        if (pb <= max_lds && e == hipSuccess) {
+            b->partial = pb;
+            b->mode = 1;
+            b->shm = pb;
+        } else {
+            (void)hipGetLastError();
+            b->partial = 0;
+            b->mode = 0;
+            b->shm = 0;
+        }
+    }
     nets.push_back(std::move(b));
     return PF_OK;
 }
 
-pf_status pf_batch::launch_all() {
-    BHIP(hipEventRecord(ev_fork, st[0]));
-    BHIP(hipStreamWaitEvent(st[1], ev_fork, 0));
-    k_zero_int<<<1, 64, 0, st[0]>>>(d_counter, (int)nets.size());
-    BHIP(hipEventRecord(ev_mid, st[0]));
-    BHIP(hipStreamWaitEvent(st[1], ev_mid, 0));
+// resident workgroups per CU of k_batch<T, mode> at this block size
+pf_status pf_batch::occupancy(const BatchNet &nb, int mode, int block, int *per_cu) {
     const bool mixed = opt.precision != PF_FP64;
-    for (size_t i = 0; i < nets.size(); i++) {
+    const size_t shm = mode == 2 ? (size_t)nb.ws : mode == 1 ? (size_t)nb.partial : 0;
+    int pc = 0;
+    hipError_t e;
+    if (mixed) {
+        if (mode == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<float, 2>, block, shm);
+        else if (mode == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<float, 1>, block, shm);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<float, 0>, block, shm);
+    } else {
+        if (mode == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<double, 2>, block, shm);
+        else if (mode == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<double, 1>, block, shm);
+        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<double, 0>, block, shm);
+    }
+    BHIP(e);
+    *per_cu = pc;
+    return PF_OK;
+}
+
+// launch shape of one topology: mode, workgroup size and workgroups per CU
+// (capped by occupancy); grows the global workspace when needed.
+pf_status pf_batch::configure(BatchNet &nb, int mode, int block, int per_cu) {
+    if (mode == 2 && !nb.lds) return PF_INVALID_ARG;
+    if (mode == 1 && (nb.lds || nb.partial == 0)) return PF_INVALID_ARG;
+    if (mode == 0 && nb.lds) return PF_INVALID_ARG;
+    if (block < 64 || block > 1024 || block % 64) return PF_INVALID_ARG;
+    int occ = 0;
+    BCHK(occupancy(nb, mode, block, &occ));
+    if (occ < 1) return PF_INVALID_ARG;
+    per_cu = per_cu <= 0 ? occ : std::min(per_cu, occ);
+    nb.mode = mode;
+    nb.block = block;
+    nb.per_cu = per_cu;
+    nb.shm = mode == 2 ? nb.ws : mode == 1 ? nb.partial : 0;
+    nb.grid = std::max(1, std::min<int>((int)nb.scen.size(), per_cu * cus));
+    if (mode != 2) {
+        const size_t need = (size_t)nb.grid * (size_t)nb.ws;
+        if (need > nb.ws_cap) {
+            char *p = nullptr;
+            BCHK(alloc(&p, need));
+            nb.d_ws = p;
+            nb.ws_cap = need;
+        }
+    }
+    return PF_OK;
+}
+
+pf_status pf_batch::launch_net(size_t i, hipStream_t s) {
+    const bool mixed = opt.precision != PF_FP64;
+    {
         BatchNet &b = *nets[i];
-        if (b.scen.empty()) continue;
+        if (b.scen.empty()) return PF_OK;
         BatchJob J{};
         J.nscen = (int)b.scen.size();
         J.scen = b.d_scen;
@@ -708,17 +910,28 @@ pf_status pf_batch::launch_all() {
         J.refine = mixed ? opt.refine_steps : 0;
         J.tol = opt.tolerancia;
         J.diverge = 1e6;
-        hipStream_t s = st[i & 1];
-        size_t shm = b.lds ? (size_t)b.ws : 0;
+        const size_t shm = (size_t)b.shm;
         if (mixed) {
-            if (b.lds) k_batch<float, true><<<b.grid, b.block, shm, s>>>(b.tp, J);
-            else k_batch<float, false><<<b.grid, b.block, 0, s>>>(b.tp, J);
+            if (b.mode == 2) k_batch<float, 2><<<b.grid, b.block, shm, s>>>(b.tp, J);
+            else if (b.mode == 1) k_batch<float, 1><<<b.grid, b.block, shm, s>>>(b.tp, J);
+            else k_batch<float, 0><<<b.grid, b.block, 0, s>>>(b.tp, J);
         } else {
-            if (b.lds) k_batch<double, true><<<b.grid, b.block, shm, s>>>(b.tp, J);
-            else k_batch<double, false><<<b.grid, b.block, 0, s>>>(b.tp, J);
+            if (b.mode == 2) k_batch<double, 2><<<b.grid, b.block, shm, s>>>(b.tp, J);
+            else if (b.mode == 1) k_batch<double, 1><<<b.grid, b.block, shm, s>>>(b.tp, J);
+            else k_batch<double, 0><<<b.grid, b.block, 0, s>>>(b.tp, J);
         }
         BHIP(hipGetLastError());
     }
+    return PF_OK;
+}
+
+pf_status pf_batch::launch_all() {
+    BHIP(hipEventRecord(ev_fork, st[0]));
+    BHIP(hipStreamWaitEvent(st[1], ev_fork, 0));
+    k_zero_int<<<1, 64, 0, st[0]>>>(d_counter, (int)nets.size());
+    BHIP(hipEventRecord(ev_mid, st[0]));
+    BHIP(hipStreamWaitEvent(st[1], ev_mid, 0));
+    for (size_t i = 0; i < nets.size(); i++) BCHK(launch_net(i, st[i & 1]));
     BHIP(hipEventRecord(ev_join, st[1]));
     BHIP(hipStreamWaitEvent(st[0], ev_join, 0));
     return PF_OK;
@@ -796,20 +1009,13 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
         BCHK(b->alloc(&b->d_it, n_scen));
         BCHK(b->alloc(&b->d_cv, n_scen));
         BCHK(b->upload(&b->d_off, b->off));
-        const bool mixed = b->opt.precision != PF_FP64;
         for (auto &nbp : b->nets) {
             BatchNet &nb = *nbp;
             if (nb.scen.empty()) continue;
             BCHK(b->upload(&nb.d_scen, nb.scen));
-            int per_cu = 1;
-            if (nb.lds) {
-                hipError_t e;
-                if (mixed) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<float, true>, nb.block, nb.ws);
-                else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<double, true>, nb.block, nb.ws);
-                if (e != hipSuccess || per_cu < 1) per_cu = 1;
-            }
-            nb.grid = std::max(1, std::min<int>((int)nb.scen.size(), per_cu * b->cus));
-            if (!nb.lds) BCHK(b->alloc(&nb.d_ws, (size_t)nb.grid * nb.ws));
+            // default shape: as many resident workgroups as fit for LDS
+            // topologies, one per CU otherwise (pf_batch_autotune refines it)
+            BCHK(b->configure(nb, nb.mode, nb.block, nb.lds ? 0 : 1));
         }
     }
     const size_t tot = (size_t)b->total;
@@ -829,12 +1035,7 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
     } else {
         b->d_V0 = b->d_T0 = nullptr;
     }
-    if (had_warm != (b->d_V0 != nullptr) && b->gexec) {   // kernel arguments change: capture again
-        (void)hipGraphExecDestroy(b->gexec);
-        (void)hipGraphDestroy(b->graph);
-        b->gexec = nullptr;
-        b->graph = nullptr;
-    }
+    if (had_warm != (b->d_V0 != nullptr)) b->drop_graph();   // kernel arguments change: capture again
     return PF_OK;
 }
 
@@ -858,6 +1059,141 @@ pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms) {
     float ms = 0;
     BHIP(hipEventElapsedTime(&ms, b->t0, b->t1));
     if (elapsed_ms) *elapsed_ms = ms;
+    return PF_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+// For every topology of the current scenario set: time one launch of its
+// persistent kernel (alone, HIP events on one stream) for each candidate
+// launch shape -- kernel MODE (working set in LDS / partly in LDS / global),
+// workgroup size, and for LDS topologies the resident workgroups per CU --
+// keep the fastest, and remember it in a text cache keyed by the device, the
+// topology's sizes, the numeric options and the scenario count, so later
+// runs apply it without measuring.
+namespace {
+
+std::string tune_key(const pf_batch &b, const BatchNet &nb, const char *arch) {
+    char k[512];
+    std::snprintf(k, sizeof k, "%s cus=%d nB=%d n=%d nnzL=%d nlev=%d nPV=%d prec=%d refine=%d lim=%d maxit=%d nscen=%zu",
+                  arch, b.cus, nb.net.nB, nb.sym.n, nb.sym.nnzL, nb.sym.nlev, nb.net.nPV, (int)b.opt.precision,
+                  b.opt.refine_steps, b.opt.lim_inj_reat ? 1 : 0, b.opt.no_max_iter, nb.scen.size());
+    return k;
+}
+
+bool cache_lookup(const char *path, const std::string &key, int *mode, int *block, int *per_cu) {
+    if (!path) return false;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[1024];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        char *tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        int m, bl, pc;
+        if (key == line && std::sscanf(tab + 1, "%d %d %d", &m, &bl, &pc) == 3) {
+            *mode = m; *block = bl; *per_cu = pc;
+            found = true;   // the last entry for a key wins
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void cache_store(const char *path, const std::string &key, int mode, int block, int per_cu, float ms) {
+    if (!path) return;
+    FILE *f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s\t%d %d %d %.4f\n", key.c_str(), mode, block, per_cu, ms);
+    std::fclose(f);
+}
+
+}  // namespace
+
+pf_status pf_batch_autotune(pf_batch *b, const char *cache_path, int reps, int *n_measured) {
+    if (!b || !b->n_scen || reps < 0) return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st[0]));
+    BHIP(hipStreamSynchronize(b->st[1]));
+    hipDeviceProp_t prop;
+    BHIP(hipGetDeviceProperties(&prop, b->device));
+    const int nrep = std::max(1, reps);
+    int measured = 0;
+    b->drop_graph();
+    for (size_t i = 0; i < b->nets.size(); i++) {
+        BatchNet &nb = *b->nets[i];
+        if (nb.scen.empty()) continue;
+        const std::string key = tune_key(*b, nb, prop.gcnArchName);
+        int m = 0, bl = 0, pc = 0;
+        if (cache_lookup(cache_path, key, &m, &bl, &pc) && b->configure(nb, m, bl, pc) == PF_OK) continue;
+        struct Cand { int mode, block, per_cu; };
+        std::vector<Cand> cands;
+        if (nb.lds) {
+            for (int block : {64, 128, 256, 512, 1024}) {
+                int occ = 0;
+                if (b->occupancy(nb, 2, block, &occ) != PF_OK || occ < 1) continue;
+                cands.push_back({2, block, occ});
+                if (occ > 2) cands.push_back({2, block, occ / 2});
+            }
+        } else {
+            for (int mode : {1, 0}) {
+                if (mode == 1 && nb.partial == 0) continue;
+                for (int block : {256, 512, 1024}) {
+                    int occ = 0;
+                    if (b->occupancy(nb, mode, block, &occ) != PF_OK || occ < 1) continue;
+                    cands.push_back({mode, block, 1});
+                    if (occ >= 2) cands.push_back({mode, block, 2});
+                }
+            }
+        }
+        float best = 1e30f;
+        Cand pick{nb.mode, nb.block, nb.per_cu};
+        for (const Cand &c : cands) {
+            BCHK(b->configure(nb, c.mode, c.block, c.per_cu));
+            float t = 1e30f;
+            for (int r = 0; r <= nrep; r++) {   // the first launch is a warm-up
+                k_zero_int<<<1, 64, 0, b->st[0]>>>(b->d_counter, (int)b->nets.size());
+                BHIP(hipEventRecord(b->t0, b->st[0]));
+                BCHK(b->launch_net(i, b->st[0]));
+                BHIP(hipEventRecord(b->t1, b->st[0]));
+                BHIP(hipEventSynchronize(b->t1));
+                float ms = 0;
+                BHIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+                if (r > 0) t = std::min(t, ms);
+            }
+            measured++;
+            if (t < best) {
+                best = t;
+                pick = c;
+            }
+        }
+        BCHK(b->configure(nb, pick.mode, pick.block, pick.per_cu));
+        cache_store(cache_path, key, pick.mode, pick.block, pick.per_cu, best);
+    }
+    b->drop_graph();   // launch shapes changed: the next solve captures again
+    if (n_measured) *n_measured = measured;
+    return PF_OK;
+}
+
+pf_status pf_batch_set_shape(pf_batch *b, int net, int mode, int block, int per_cu) {
+    if (!b || !b->n_scen || net < 0 || net >= (int)b->nets.size()) return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    BatchNet &nb = *b->nets[net];
+    if (nb.scen.empty()) return PF_INVALID_ARG;
+    BHIP(hipStreamSynchronize(b->st[0]));
+    BHIP(hipStreamSynchronize(b->st[1]));
+    BCHK(b->configure(nb, mode, block, per_cu));
+    b->drop_graph();
+    return PF_OK;
+}
+
+pf_status pf_batch_get_shape(pf_batch *b, int net, int *mode, int *block, int *grid, long long *lds_bytes) {
+    if (!b || net < 0 || net >= (int)b->nets.size()) return PF_INVALID_ARG;
+    const BatchNet &nb = *b->nets[net];
+    if (mode) *mode = nb.mode;
+    if (block) *block = nb.block;
+    if (grid) *grid = nb.grid;
+    if (lds_bytes) *lds_bytes = nb.shm;
     return PF_OK;
 }
 

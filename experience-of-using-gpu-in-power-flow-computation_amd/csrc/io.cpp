@@ -216,6 +216,7 @@ void options_default(pf_options &o) {
     o.precision = PF_FP64;
     o.refine_steps = 2;
     o.use_graph = 0;
+    o.lu_blocked = 1;
 }
 
 namespace {
@@ -291,6 +292,8 @@ pf_status options_parse(const std::string &s, int cpu_layout, pf_options &o) {
         o.refine_steps = std::atoi(v.c_str());
     if (s.find("grafo") != std::string::npos && value_of(s, "grafo", 2, v))
         o.use_graph = (v == "ON") ? 1 : 0;
+    if (s.find("lu_blocos") != std::string::npos && value_of(s, "lu_blocos", 2, v))
+        o.lu_blocked = (v == "OFF") ? 0 : 1;
     return PF_OK;
 }
 

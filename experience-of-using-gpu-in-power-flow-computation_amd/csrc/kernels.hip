@@ -314,7 +314,9 @@ __device__ __forceinline__ void factor_node(const DevSym &Y, T *blk, int j, int 
             dinv.d = __shfl(di.d, 0);
         }
         if (t < nd && t > 0) {
-            store_b4(blk, L0 + p, mul(a, dinv));
+            const B4<T> l = mul(a, dinv);
+            store_b4(blk, L0 + p, l);
+            store_b4(blk, U0 + Y.nnzL + Y.lr_pos[p], l);
             store_b4(blk, U0 + p, u);
         }
     }
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl
     if (gate && !ctl->step) return;
     const int idx = blockIdx.x * 4 + (int)threadIdx.x / 64;
     if (idx >= count) return;
-    factor_node(Y, blk, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 63, &ctl->singular);
+    factor_node(Y, blk, (l0 + idx), (int)threadIdx.x & 63, &ctl->singular);
 }
 
 template <typename T>
@@ -335,6 +337,7 @@ __global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl
         coop_factor_level<T, 16>(Y, blk, l, &ctl->singular);
         __syncthreads();
     }
+    if (Y.ch_m > 0) chain_factor<T, 16, kChainRegs, kChainMax>(Y, blk, &ctl->singular);
 }
 
 // ----------------------------------------------------- triangular solves
@@ -343,7 +346,7 @@ template <typename T>
 __device__ __forceinline__ void fwd_node(const DevSym &Y, const T *blk, double *x, int j, int sl) {
     double a0 = 0, a1 = 0;
     for (int q = Y.rs_ptr[j] + sl; q < Y.rs_ptr[j + 1]; q += 16) {
-        B4<T> L = load_b4(blk, Y.n + Y.rs_p[q]);
+        B4<T> L = load_b4(blk, Y.n + 2 * Y.nnzL + q);
         const int k = Y.rs_k[q];
         const double x0 = x[2 * k], x1 = x[2 * k + 1];
         a0 += (double)L.a * x0 + (double)L.b * x1;
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(256) k_fwd_level(DevSym Y, const T *blk, doubl
     if (gate && !ctl->step) return;
     const int idx = blockIdx.x * 16 + (int)threadIdx.x / 16;
     if (idx >= count) return;
-    fwd_node(Y, blk, x, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 15);
+    fwd_node(Y, blk, x, (l0 + idx), (int)threadIdx.x & 15);
 }
 
 template <typename T>
@@ -397,21 +400,91 @@ __global__ void __launch_bounds__(256) k_bwd_level(DevSym Y, const T *blk, doubl
     if (gate && !ctl->step) return;
     const int idx = blockIdx.x * 16 + (int)threadIdx.x / 16;
     if (idx >= count) return;
-    bwd_node(Y, blk, x, Y.lvl_nodes[l0 + idx], (int)threadIdx.x & 15);
+    bwd_node(Y, blk, x, (l0 + idx), (int)threadIdx.x & 15);
 }
 
-// narrow levels: one wavefront per node
+// Levels below the tail in one launch: workgroup g walks the levels of its
+// own group of subtrees (every update of a node comes from its descendants,
+// so the groups never wait on each other).
+template <typename T>
+__global__ void __launch_bounds__(256) k_factor_parts(DevSym Y, T *blk, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63, nl = Y.part_levels;
+    const int *pp = Y.part_ptr + (size_t)blockIdx.x * nl;
+    for (int l = 0; l < nl; l++) {
+        const int s = pp[l], e = pp[l + 1];
+        if (s == e) continue;
+        for (int i = s + w; i < e; i += 4) factor_node(Y, blk, Y.part_nodes[i], lane, &ctl->singular);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    const int g = (int)threadIdx.x / 16, lane = (int)threadIdx.x & 15, nl = Y.part_levels;
+    const int *pp = Y.part_ptr + (size_t)blockIdx.x * nl;
+    for (int l = 0; l < nl; l++) {
+        const int s = pp[l], e = pp[l + 1];
+        if (s == e) continue;
+        for (int i = s + g; i < e; i += 16) fwd_node(Y, blk, x, Y.part_nodes[i], lane);
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    const int g = (int)threadIdx.x / 16, lane = (int)threadIdx.x & 15, nl = Y.part_levels;
+    const int *pp = Y.part_ptr + (size_t)blockIdx.x * nl;
+    for (int l = nl - 1; l >= 0; l--) {
+        const int s = pp[l], e = pp[l + 1];
+        if (s == e) continue;
+        for (int i = s + g; i < e; i += 16) bwd_node(Y, blk, x, Y.part_nodes[i], lane);
+        __syncthreads();
+    }
+}
+
+// narrow levels: one wavefront per node.  The first entry of a wave's first
+// node on the next level (index and factor block, independent of x) is
+// loaded before the current level's arithmetic and barrier.
 template <typename T>
 __global__ void __launch_bounds__(1024) k_fwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
     const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
+    const int LR0 = Y.n + 2 * Y.nnzL;
+    auto fetch = [&](int l, int &k, B4<T> &L) {
+        k = -1;
+        const int s = Y.lvl_ptr[l];
+        if (w < Y.lvl_ptr[l + 1] - s) {
+            const int j = s + w, q = Y.rs_ptr[j] + lane;
+            if (q < Y.rs_ptr[j + 1]) {
+                k = Y.rs_k[q];
+                L = load_b4(blk, LR0 + q);
+            }
+        }
+    };
+    int pk = -1;
+    B4<T> pL{0, 0, 0, 0};
+    if (lv0 < lv1) fetch(lv0, pk, pL);
     for (int l = lv0; l < lv1; l++) {
+        int nk = -1;
+        B4<T> nL{0, 0, 0, 0};
+        if (l + 1 < lv1) fetch(l + 1, nk, nL);
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
         for (int idx = w; idx < c; idx += 16) {
-            const int j = Y.lvl_nodes[s + idx];
+            const int j = (s + idx);
             double a0 = 0, a1 = 0;
-            for (int q = Y.rs_ptr[j] + lane; q < Y.rs_ptr[j + 1]; q += 64) {
-                B4<T> L = load_b4(blk, Y.n + Y.rs_p[q]);
+            int q = Y.rs_ptr[j] + lane;
+            if (idx == w) {
+                if (pk >= 0) {
+                    a0 += (double)pL.a * x[2 * pk] + (double)pL.b * x[2 * pk + 1];
+                    a1 += (double)pL.c * x[2 * pk] + (double)pL.d * x[2 * pk + 1];
+                }
+                q += 64;
+            }
+            for (; q < Y.rs_ptr[j + 1]; q += 64) {
+                B4<T> L = load_b4(blk, LR0 + q);
                 const int k = Y.rs_k[q];
                 const double x0 = x[2 * k], x1 = x[2 * k + 1];
                 a0 += (double)L.a * x0 + (double)L.b * x1;
@@ -425,7 +498,10 @@ __global__ void __launch_bounds__(1024) k_fwd_tail(DevSym Y, const T *blk, doubl
             }
         }
         __syncthreads();
+        pk = nk;
+        pL = nL;
     }
+    if (Y.ch_m > 0) chain_solve<T, 16, kChainMax>(Y, blk, x);
 }
 
 template <typename T>
@@ -433,12 +509,38 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
     if (gate && !ctl->step) return;
     const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
     const int U0 = Y.n + Y.nnzL;
+    auto fetch = [&](int l, int &r, B4<T> &U, B4<T> &D) {
+        r = -1;
+        const int s = Y.lvl_ptr[l];
+        if (w < Y.lvl_ptr[l + 1] - s) {
+            const int j = s + w, p = Y.colptr[j] + lane;
+            if (lane == 0) D = load_b4(blk, j);
+            if (p < Y.colptr[j + 1]) {
+                r = Y.rowind[p];
+                U = load_b4(blk, U0 + p);
+            }
+        }
+    };
+    int pr = -1;
+    B4<T> pU{0, 0, 0, 0}, pD{0, 0, 0, 0};
+    if (lv0 < lv1) fetch(lv1 - 1, pr, pU, pD);
     for (int l = lv1 - 1; l >= lv0; l--) {
+        int nr = -1;
+        B4<T> nU{0, 0, 0, 0}, nD{0, 0, 0, 0};
+        if (l > lv0) fetch(l - 1, nr, nU, nD);
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
         for (int idx = w; idx < c; idx += 16) {
-            const int j = Y.lvl_nodes[s + idx];
+            const int j = (s + idx);
             double a0 = 0, a1 = 0;
-            for (int p = Y.colptr[j] + lane; p < Y.colptr[j + 1]; p += 64) {
+            int p = Y.colptr[j] + lane;
+            if (idx == w) {
+                if (pr >= 0) {
+                    a0 += (double)pU.a * x[2 * pr] + (double)pU.b * x[2 * pr + 1];
+                    a1 += (double)pU.c * x[2 * pr] + (double)pU.d * x[2 * pr + 1];
+                }
+                p += 64;
+            }
+            for (; p < Y.colptr[j + 1]; p += 64) {
                 B4<T> U = load_b4(blk, U0 + p);
                 const int i = Y.rowind[p];
                 const double x0 = x[2 * i], x1 = x[2 * i + 1];
@@ -449,12 +551,15 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
             a1 = wave_sum(a1);
             if (lane == 0) {
                 const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
-                B4<T> D = load_b4(blk, j);
+                const B4<T> D = (idx == w) ? pD : load_b4(blk, j);
                 x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
                 x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
             }
         }
         __syncthreads();
+        pr = nr;
+        pU = nU;
+        pD = nD;
     }
 }
 
@@ -645,6 +750,12 @@ template __global__ void k_jfill<double>(DevNet, DevState, DevSym, double *, int
 template __global__ void k_jfill<float>(DevNet, DevState, DevSym, float *, int);
 template __global__ void k_factor_level<double>(DevSym, double *, Ctl *, int, int, int);
 template __global__ void k_factor_level<float>(DevSym, float *, Ctl *, int, int, int);
+template __global__ void k_factor_parts<double>(DevSym, double *, Ctl *, int);
+template __global__ void k_factor_parts<float>(DevSym, float *, Ctl *, int);
+template __global__ void k_fwd_parts<double>(DevSym, const double *, double *, Ctl *, int);
+template __global__ void k_fwd_parts<float>(DevSym, const float *, double *, Ctl *, int);
+template __global__ void k_bwd_parts<double>(DevSym, const double *, double *, Ctl *, int);
+template __global__ void k_bwd_parts<float>(DevSym, const float *, double *, Ctl *, int);
 template __global__ void k_factor_tail<double>(DevSym, double *, Ctl *, int, int, int);
 template __global__ void k_factor_tail<float>(DevSym, float *, Ctl *, int, int, int);
 template __global__ void k_fwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "pf_internal.hpp"
+
 namespace pf {
 
 constexpr double kFolga = 1e-4;   // tol_limInjReat (CPU/global.cpp:11)
@@ -55,6 +57,12 @@ struct DevState {
     Ctl *ctl;
 };
 
+// trailing dense chain of the single-network solver: at most kChainMax
+// nodes (pf_internal.hpp), S held in kChainRegs register blocks per thread
+// of a 1024-thread workgroup
+constexpr int kChainRegs = 10;
+static_assert(kChainMax * kChainMax <= 1024 * kChainRegs, "dense chain does not fit the registers");
+
 // Symbolic factor program on the device.
 struct DevSym {
     int n, nnzL, nslot, nlev;
@@ -62,6 +70,13 @@ struct DevSym {
     const int *lvl_ptr, *lvl_nodes;
     const int *fd_ptr, *fc_ptr, *fc_a, *fc_b;
     const int *rs_ptr, *rs_k, *rs_p;
+    const int *lr_pos;                      // L position -> row-ordered copy (block n + 2 nnzL + q)
+    // trailing dense chain [ch_c0, n) on levels [nlev_s, nlev); ch_m = 0: none
+    int ch_m, ch_c0, nlev_s;
+    const int *ch_split;
+    // groups of whole subtrees below the tail, one workgroup each
+    int n_parts, part_levels;
+    const int *part_ptr, *part_nodes;
     const int *slot_e;
     const int *ld_ptr, *ld_node, *ld_d;     // flattened destinations per level
 };

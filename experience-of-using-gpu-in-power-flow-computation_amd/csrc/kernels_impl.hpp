@@ -108,7 +108,7 @@ __device__ void coop_factor_level(const DevSym &Y, T *blk, int l, int *bad) {
     __syncthreads();
     const int n0 = Y.lvl_ptr[l], nn = Y.lvl_ptr[l + 1] - n0;
     for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-        const int j = Y.lvl_nodes[n0 + i];
+        const int j = (n0 + i);
         B4<T> D = load_b4(blk, j);
         T det = D.a * D.d - D.b * D.c;
         if (!(det != (T)0) || !isfinite((double)det)) {
@@ -123,8 +123,208 @@ __device__ void coop_factor_level(const DevSym &Y, T *blk, int l, int *bad) {
         const int t = d - Y.fd_ptr[j];
         if (t == 0) continue;
         const int p = Y.colptr[j] + t - 1;
-        store_b4(blk, L0 + p, mul(load_b4(blk, L0 + p), load_b4(blk, j)));
+        const B4<T> l = mul(load_b4(blk, L0 + p), load_b4(blk, j));
+        store_b4(blk, L0 + p, l);
+        store_b4(blk, U0 + Y.nnzL + Y.lr_pos[p], l);
     }
+}
+
+// ------------------------------------------------- trailing dense chain
+// The top of a power-network elimination tree ends in a chain of single
+// nodes whose columns are full (the separator between the network's areas).
+// Instead of one level per node, the chain is one dense block: its Schur
+// complement S = A_CC - L_CN U_NC (the updates from every node before the
+// chain, all of them independent) and then S^-1 by Gauss-Jordan on 2x2
+// blocks, without pivoting like the rest of the factorisation.  The forward
+// and backward solves of the chain collapse into
+//     x_C = S^-1 (b_C - L_CN y_N)
+// so a solve costs two barriers there instead of two levels per node.
+
+// S = chain block minus the updates of the nodes before the chain, written
+// row-major (ch_m x ch_m blocks) at block n + 3 nnzL.  Groups of G lanes
+// per destination block; every destination is independent.  The caller
+// synchronises before (every node before the chain factored).
+template <typename T, int G>
+__device__ void chain_schur(const DevSym &Y, T *blk) {
+    const int m = Y.ch_m, c0 = Y.ch_c0;
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL, SB = Y.n + 3 * Y.nnzL;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    {
+        const int e0 = Y.ld_ptr[Y.nlev_s], ne = Y.ld_ptr[Y.nlev] - e0, dbase = Y.fd_ptr[c0];
+        const int grp = tid / G, lane = tid % G, ng = bd / G;
+        for (int e = grp; e < ne; e += ng) {
+            const int j = Y.ld_node[e0 + e], d = Y.ld_d[e0 + e];
+            const int t = d - Y.fd_ptr[j], a = j - c0, qend = Y.ch_split[d - dbase];
+            B4<T> acc{0, 0, 0, 0}, uacc{0, 0, 0, 0};
+            if (t == 0) {
+                for (int q = Y.fc_ptr[d] + lane; q < qend; q += G)
+                    mma2(acc, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
+            } else {
+                for (int q = Y.fc_ptr[d] + lane; q < qend; q += G) {
+                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
+                    mma2(acc, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
+                    mma2(uacc, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
+                }
+            }
+            acc = group_sum<T, G>(acc);
+            uacc = group_sum<T, G>(uacc);
+            if (lane == 0) {
+                if (t == 0) {
+                    const B4<T> D = load_b4(blk, j);
+                    store_b4(blk, SB + a * m + a, B4<T>{D.a - acc.a, D.b - acc.b, D.c - acc.c, D.d - acc.d});
+                } else {
+                    const int p = Y.colptr[j] + t - 1, i = a + t;   // row node j + t
+                    const B4<T> Lb = load_b4(blk, L0 + p), Ub = load_b4(blk, U0 + p);
+                    store_b4(blk, SB + i * m + a, B4<T>{Lb.a - acc.a, Lb.b - acc.b, Lb.c - acc.c, Lb.d - acc.d});
+                    store_b4(blk, SB + a * m + i,
+                             B4<T>{Ub.a - uacc.a, Ub.b - uacc.b, Ub.c - uacc.c, Ub.d - uacc.d});
+                }
+            }
+        }
+    }
+}
+
+// 2x2 pivot inverse; a zero or non-finite determinant raises *bad (thread 0)
+template <typename T>
+__device__ __forceinline__ B4<T> pivot_inverse(const B4<T> &pv, int *bad) {
+    T det = pv.a * pv.d - pv.b * pv.c;
+    if (!(det != (T)0) || !isfinite((double)det)) {
+        if (threadIdx.x == 0) atomicOr(bad, 1);
+        det = (T)1;
+    }
+    return B4<T>{pv.d / det, -pv.b / det, -pv.c / det, pv.a / det};
+}
+
+// One Gauss-Jordan update of block (i, j) at step k, from the step's pivot
+// inverse P and the old row k (rk = S_kj) and column k (ci = S_ik):
+//   S_kk <- P; S_kj <- P S_kj; S_ik <- -S_ik P; S_ij <- S_ij - S_ik (P S_kj).
+template <typename T>
+__device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B4<T> &P, const B4<T> &rk,
+                                          const B4<T> &ci) {
+    if (i == k) {
+        s = (j == k) ? P : mul(P, rk);
+    } else if (j == k) {
+        const B4<T> c = mul(ci, P);
+        s = B4<T>{-c.a, -c.b, -c.c, -c.d};
+    } else {
+        mms(s, ci, mul(P, rk));
+    }
+}
+
+// Forms S and overwrites it with S^-1.  S lives in registers during the
+// elimination: thread t owns blocks t + r*blockDim, r < R (requires
+// ch_m^2 <= R * blockDim); row k and column k go through LDS (double
+// buffered: one barrier per step).  The caller synchronises before and after.
+template <typename T, int G, int R, int MAXM>
+__device__ void chain_factor(const DevSym &Y, T *blk, int *bad) {
+    __shared__ B4<T> s_row[2][MAXM], s_col[2][MAXM];
+    const int m = Y.ch_m, mm = m * m, SB = Y.n + 3 * Y.nnzL;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    chain_schur<T, G>(Y, blk);
+    __syncthreads();
+    B4<T> reg[R];
+    int ri[R], rj[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int q = tid + r * bd;
+        ri[r] = q < mm ? q / m : -1;
+        rj[r] = q < mm ? q - ri[r] * m : -1;
+        reg[r] = q < mm ? load_b4(blk, SB + q) : B4<T>{0, 0, 0, 0};
+    }
+    for (int k = 0; k < m; k++) {
+        const int b = k & 1;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (ri[r] == k) s_row[b][rj[r]] = reg[r];
+            if (rj[r] == k) s_col[b][ri[r]] = reg[r];
+        }
+        __syncthreads();
+        const B4<T> P = pivot_inverse(s_row[b][k], bad);
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (ri[r] >= 0) gj_update(reg[r], ri[r], rj[r], k, P, s_row[b][rj[r]], s_col[b][ri[r]]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (ri[r] >= 0) store_b4(blk, SB + tid + r * bd, reg[r]);
+}
+
+// Same, with S updated where it is stored (LDS or a global slice): for a
+// workgroup with no registers to spare.  Two barriers per step.
+template <typename T, int G, int MAXM>
+__device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad) {
+    __shared__ B4<T> s_row[MAXM], s_col[MAXM];
+    const int m = Y.ch_m, mm = m * m, SB = Y.n + 3 * Y.nnzL;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    chain_schur<T, G>(Y, blk);
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        for (int t = tid; t < m; t += bd) {
+            s_row[t] = load_b4(blk, SB + k * m + t);
+            s_col[t] = load_b4(blk, SB + t * m + k);
+        }
+        __syncthreads();
+        const B4<T> P = pivot_inverse(s_row[k], bad);
+        for (int q = tid; q < mm; q += bd) {
+            const int i = q / m, j = q - i * m;
+            B4<T> s = load_b4(blk, SB + q);
+            gj_update(s, i, j, k, P, s_row[j], s_col[i]);
+            store_b4(blk, SB + q, s);
+        }
+        __syncthreads();
+    }
+}
+
+// Chain part of a solve, after the forward sweep over the levels before the
+// chain: x_C <- S^-1 (x_C - L_CN x_N), where x_C still holds the right-hand
+// side.  The backward sweep then starts below the chain.  Two barriers.
+template <typename T, int G, int MAXM>
+__device__ void chain_solve(const DevSym &Y, const T *blk, double *x) {
+    __shared__ double s_r[2 * MAXM];
+    const int m = Y.ch_m, c0 = Y.ch_c0;
+    const int LR0 = Y.n + 2 * Y.nnzL, SB = Y.n + 3 * Y.nnzL;
+    const int tid = (int)threadIdx.x, grp = tid / G, lane = tid % G, ng = (int)blockDim.x / G;
+    for (int a = grp; a < m; a += ng) {
+        const int c = c0 + a;
+        double a0 = 0, a1 = 0;
+        for (int q = Y.rs_ptr[c] + lane; q < Y.rs_ptr[c + 1]; q += G) {
+            const int k = Y.rs_k[q];
+            if (k >= c0) break;                 // row entries ascend in k
+            const B4<T> L = load_b4(blk, LR0 + q);
+            const double x0 = x[2 * k], x1 = x[2 * k + 1];
+            a0 += (double)L.a * x0 + (double)L.b * x1;
+            a1 += (double)L.c * x0 + (double)L.d * x1;
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            a0 += __shfl_xor(a0, off, G);
+            a1 += __shfl_xor(a1, off, G);
+        }
+        if (lane == 0) {
+            s_r[2 * a] = x[2 * c] - a0;
+            s_r[2 * a + 1] = x[2 * c + 1] - a1;
+        }
+    }
+    __syncthreads();
+    for (int a = grp; a < m; a += ng) {
+        double a0 = 0, a1 = 0;
+        for (int b = lane; b < m; b += G) {
+            const B4<T> S = load_b4(blk, SB + a * m + b);
+            const double r0 = s_r[2 * b], r1 = s_r[2 * b + 1];
+            a0 += (double)S.a * r0 + (double)S.b * r1;
+            a1 += (double)S.c * r0 + (double)S.d * r1;
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            a0 += __shfl_xor(a0, off, G);
+            a1 += __shfl_xor(a1, off, G);
+        }
+        if (lane == 0) {
+            x[2 * (c0 + a)] = a0;
+            x[2 * (c0 + a) + 1] = a1;
+        }
+    }
+    __syncthreads();
 }
 
 // Smith's complex division (CACM Algorithm 116, 1962): (ar + j ai) / (br + j bi)

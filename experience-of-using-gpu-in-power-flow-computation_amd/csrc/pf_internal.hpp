@@ -23,6 +23,9 @@ struct pf_network {
 
 namespace pf {
 
+// longest trailing dense chain the single-network solver factors as one block
+constexpr int kChainMax = 101;
+
 // Ybus pattern (host, once per network): diagonal + both directions of every
 // branch, columns sorted; plus, per branch, the four CSR slots its pi-model
 // terms land in.  Values are assembled on the GPU.
@@ -47,7 +50,7 @@ struct Symbolic {
     std::vector<int> colptr, rowind;   // struct(L(:,j)), ascending rows > j
     std::vector<int> parent, level;
     int nlev = 0;
-    std::vector<int> lvl_ptr, lvl_nodes;   // nodes grouped by level
+    std::vector<int> lvl_ptr, lvl_nodes;   // nodes grouped by level (the identity after analyze)
     int tail_start = 0;         // first level processed by the single-workgroup tail
     // factor program: per node, destinations (diag first, then column entries)
     std::vector<int> fd_ptr;    // [n+1] into destination list
@@ -59,13 +62,45 @@ struct Symbolic {
     std::vector<int> ld_ptr, ld_node, ld_d;
     // forward-solve rows: per node j, k in rowstruct(j) and position p_jk
     std::vector<int> rs_ptr, rs_k, rs_p;
+    // the factor also stores L row by row for the forward solve: L position
+    // p is copied to block n + 2 nnzL + lr_pos[p] (lr_pos inverts rs_p)
+    std::vector<int> lr_pos;
+    // trailing dense chain: the last ch_m nodes [ch_c0, n), one per level
+    // (levels [nlev_s, nlev)), each column holding every later chain node.
+    // Their Schur complement S (ch_m x ch_m blocks) is formed with the
+    // updates of the nodes before ch_c0 and inverted densely; S^-1 is stored
+    // row-major at block n + 3 nnzL.  ch_split[e]: for chain destination
+    // fd_ptr[ch_c0] + e, the end of its update pairs from nodes < ch_c0.
+    int ch_m = 0, ch_c0 = 0, nlev_s = 0;
+    std::vector<int> ch_split;
+    // the levels below tail_start split into groups of whole subtrees, one
+    // workgroup per group (no dependence between groups): group g owns the
+    // nodes part_nodes[part_ptr[g * tail_start + l] .. part_ptr[g * tail_start + l + 1])
+    // of level l
+    int n_parts = 0;
+    std::vector<int> part_ptr, part_nodes;
     // Jacobian slot map: block slot -> Y entry (-1: fill-in, zero)
     //  slots [0, n): diagonal, [n, n+nnzL): L, [n+nnzL, n+2nnzL): U
     std::vector<int> slot_e;
     int nslot() const { return n + 2 * nnzL; }
+    // slots + the row-ordered copy of L + the dense chain inverse
+    int nblk() const { return n + 3 * nnzL + ch_m * ch_m; }
+    // nodes numbered level by level: level l is the node range
+    // [lvl_ptr[l], lvl_ptr[l+1]); the GPU kernels rely on it
+    bool level_ordered() const {
+        for (int i = 0; i < n; i++)
+            if (lvl_nodes[i] != i) return false;
+        return true;
+    }
 };
 
-Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width);
+// chain_cap: longest trailing dense chain factored as one block (0: none)
+Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap);
+
+// Splits the levels below s.tail_start into at most max_parts groups of
+// whole subtrees, balanced by factorisation work (longest processing time
+// first).
+void partition_subtrees(Symbolic &s, int max_parts);
 
 // Minimum-degree ordering of a symmetric pattern (CSR, diagonal ignored).
 std::vector<int> min_degree(int n, const std::vector<int> &rowptr, const std::vector<int> &colind);

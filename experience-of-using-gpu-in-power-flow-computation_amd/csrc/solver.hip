@@ -132,7 +132,11 @@ pf_status pf_solver::build() {
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&ctl_host, 2 * sizeof(Ctl), hipHostMallocDefault));
     yp = build_ypattern(net);
-    sym = analyze(net, yp, kTailWidth);
+    sym = analyze(net, yp, kTailWidth, opt.lu_blocked ? kChainMax : 0);
+    if (!sym.level_ordered() || (sym.ch_m > 0 && sym.tail_start > sym.nlev_s)) {
+        g_last_error = "symbolic analysis: nodes not numbered level by level, or chain outside the tail";
+        return PF_INVALID_ARG;
+    }
     const int nB = net.nB, n = sym.n, sl = net.slack;
     // slot of every Y entry (for the reduced-layout Jacobian export)
     e2slot.assign(yp.nnz, -1);
@@ -199,6 +203,19 @@ pf_status pf_solver::build() {
     dy.nnzL = sym.nnzL;
     dy.nslot = sym.nslot();
     dy.nlev = sym.nlev;
+    dy.ch_m = sym.ch_m;
+    dy.ch_c0 = sym.ch_c0;
+    dy.nlev_s = sym.nlev_s;
+    PFCHK(upload((int **)&dy.ch_split, sym.ch_split));
+    if (opt.lu_blocked) {   // one workgroup per group of subtrees, one group per CU
+        int cus = 0;
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        partition_subtrees(sym, std::max(1, cus));
+    }
+    dy.n_parts = sym.n_parts;
+    dy.part_levels = sym.tail_start;
+    PFCHK(upload((int **)&dy.part_ptr, sym.part_ptr));
+    PFCHK(upload((int **)&dy.part_nodes, sym.part_nodes));
     PFCHK(upload((int **)&dy.colptr, sym.colptr));
     PFCHK(upload((int **)&dy.rowind, sym.rowind));
     PFCHK(upload((int **)&dy.pcol, pcol));
@@ -211,12 +228,13 @@ pf_status pf_solver::build() {
     PFCHK(upload((int **)&dy.rs_ptr, sym.rs_ptr));
     PFCHK(upload((int **)&dy.rs_k, sym.rs_k));
     PFCHK(upload((int **)&dy.rs_p, sym.rs_p));
+    PFCHK(upload((int **)&dy.lr_pos, sym.lr_pos));
     PFCHK(upload((int **)&dy.slot_e, sym.slot_e));
     PFCHK(upload((int **)&dy.ld_ptr, sym.ld_ptr));
     PFCHK(upload((int **)&dy.ld_node, sym.ld_node));
     PFCHK(upload((int **)&dy.ld_d, sym.ld_d));
-    PFCHK(alloc(&blk64, 4 * (size_t)sym.nslot()));
-    if (opt.precision == PF_MIXED) PFCHK(alloc(&blk32, 4 * (size_t)sym.nslot()));
+    PFCHK(alloc(&blk64, 4 * (size_t)sym.nblk()));
+    if (opt.precision == PF_MIXED) PFCHK(alloc(&blk32, 4 * (size_t)sym.nblk()));
 
     // Ybus on the device (once per network)
     k_ybus<<<(yp.nnz + 255) / 256, 256, 0, stream>>>(dn, yp.nnz);
@@ -281,29 +299,41 @@ pf_status pf_solver::enqueue_head(int gate) {
 template <typename T>
 pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     k_jfill<T><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk, gate);
-    for (int l = 0; l < sym.tail_start; l++) {
-        int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
-        k_factor_level<T><<<(c + 3) / 4, 256, 0, stream>>>(dy, blk, ctl, sym.lvl_ptr[l], c, gate);
+    if (sym.n_parts > 0) {
+        k_factor_parts<T><<<sym.n_parts, 256, 0, stream>>>(dy, blk, ctl, gate);
+    } else {
+        for (int l = 0; l < sym.tail_start; l++) {
+            int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
+            k_factor_level<T><<<(c + 3) / 4, 256, 0, stream>>>(dy, blk, ctl, sym.lvl_ptr[l], c, gate);
+        }
     }
-    if (sym.tail_start < sym.nlev)
-        k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev, gate);
+    if (sym.tail_start < sym.nlev)   // narrow levels below the chain, then the chain
+        k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev_s, gate);
     HIPCHK(hipGetLastError());
     return PF_OK;
 }
 
 template <typename T>
 pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
-    for (int l = 0; l < sym.tail_start; l++) {
-        int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
-        k_fwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
+    if (sym.n_parts > 0) {
+        k_fwd_parts<T><<<sym.n_parts, 256, 0, stream>>>(dy, blk, v, ctl, gate);
+    } else {
+        for (int l = 0; l < sym.tail_start; l++) {
+            int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
+            k_fwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
+        }
     }
-    if (sym.tail_start < sym.nlev) {
-        k_fwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev, gate);
-        k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev, gate);
-    }
-    for (int l = sym.tail_start - 1; l >= 0; l--) {
-        int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
-        k_bwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
+    if (sym.tail_start < sym.nlev)
+        k_fwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
+    if (sym.tail_start < sym.nlev_s)
+        k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
+    if (sym.n_parts > 0) {
+        k_bwd_parts<T><<<sym.n_parts, 256, 0, stream>>>(dy, blk, v, ctl, gate);
+    } else {
+        for (int l = sym.tail_start - 1; l >= 0; l--) {
+            int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
+            k_bwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
+        }
     }
     HIPCHK(hipGetLastError());
     return PF_OK;
@@ -351,9 +381,9 @@ pf_status pf_solver::run_hybrid_step() {
     const size_t ns = sym.nslot();
     k_jfill<double><<<(ns + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk64, 1);
     HIPCHK(hipGetLastError());
-    hblk.resize(4 * ns);
+    hblk.assign(4 * (size_t)sym.nblk(), 0.0);   // slots, then room for the dense chain
     hx.resize(2 * (size_t)sym.n);
-    PFCHK(get(hblk.data(), blk64, hblk.size()));
+    PFCHK(get(hblk.data(), blk64, 4 * ns));
     PFCHK(get(hx.data(), ds.g, hx.size()));
     if (!host_block_factor(sym, hblk.data())) {
         ctl_host[0].singular = 1;
@@ -676,8 +706,8 @@ pf_status pf_stage_lu_solve(pf_solver *s, const double *V, const double *theta,
         const size_t ns = s->sym.nslot();
         k_jfill<double><<<(ns + 255) / 256, 256, 0, s->stream>>>(s->dn, s->ds, s->dy, s->blk64, 0);
         HIPCHK(hipGetLastError());
-        std::vector<double> b(4 * ns);
-        PFCHK(s->get(b.data(), s->blk64, b.size()));
+        std::vector<double> b(4 * (size_t)s->sym.nblk());
+        PFCHK(s->get(b.data(), s->blk64, 4 * ns));
         if (!host_block_factor(s->sym, b.data())) return PF_SINGULAR;
         xn = gn;
         host_block_solve(s->sym, b.data(), xn.data());

@@ -124,24 +124,18 @@ std::vector<int> min_degree(int n, const std::vector<int> &rowptr, const std::ve
     return perm;
 }
 
-Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width) {
+namespace {
+
+// Symbolic analysis for a given elimination order perm (node j eliminates
+// block perm[j]); gp/gi is the bus graph without the slack in block numbering.
+Symbolic analyze_order(const pf_network &net, const YPattern &yp, int tail_width,
+                       const std::vector<int> &perm, const std::vector<int> &gp, const std::vector<int> &gi) {
     Symbolic s;
     const int nB = net.nB, sl = net.slack;
     const int n = nB - 1;
     s.n = n;
     auto blk = [&](int k) { return k - (k > sl ? 1 : 0); };
-    // bus graph without the slack, in block numbering
-    std::vector<int> gp(n + 1, 0), gi;
-    for (int k = 0; k < nB; k++) {
-        if (k == sl) continue;
-        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
-            int m = yp.colind[e];
-            if (m == sl || m == k) continue;
-            gi.push_back(blk(m));
-        }
-        gp[blk(k) + 1] = (int)gi.size();
-    }
-    s.perm = min_degree(n, gp, gi);
+    s.perm = perm;
     s.iperm.assign(n, 0);
     for (int j = 0; j < n; j++) s.iperm[s.perm[j]] = j;
     // upper adjacency in elimination order
@@ -217,6 +211,8 @@ Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width) {
                 fill[i]++;
             }
     }
+    s.lr_pos.resize(s.nnzL);
+    for (int q = 0; q < s.nnzL; q++) s.lr_pos[s.rs_p[q]] = q;
     // factor program
     s.fd_ptr.assign(n + 1, 0);
     s.fc_ptr.assign(1, 0);
@@ -275,6 +271,98 @@ Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width) {
     return s;
 }
 
+}  // namespace
+
+// Minimum degree, then the same order renumbered level by level (stable in
+// the minimum-degree order inside a level).  Children sit on lower levels
+// than their parents, so the renumbering is a topological order of the same
+// elimination tree: the fill is unchanged, while every level becomes a
+// contiguous range of nodes (lvl_nodes is the identity) and the factor
+// program, the row structures and the flattened destinations of a level are
+// contiguous too.
+Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap) {
+    const int nB = net.nB, sl = net.slack;
+    const int n = nB - 1;
+    auto blk = [&](int k) { return k - (k > sl ? 1 : 0); };
+    // bus graph without the slack, in block numbering
+    std::vector<int> gp(n + 1, 0), gi;
+    for (int k = 0; k < nB; k++) {
+        if (k == sl) continue;
+        for (int e = yp.rowptr[k]; e < yp.rowptr[k + 1]; e++) {
+            int m = yp.colind[e];
+            if (m == sl || m == k) continue;
+            gi.push_back(blk(m));
+        }
+        gp[blk(k) + 1] = (int)gi.size();
+    }
+    const std::vector<int> md = min_degree(n, gp, gi);
+    const Symbolic s0 = analyze_order(net, yp, tail_width, md, gp, gi);
+    std::vector<int> by_level(s0.lvl_nodes.begin(), s0.lvl_nodes.end());   // nodes grouped by level
+    std::vector<int> perm(n);
+    for (int t = 0; t < n; t++) perm[t] = md[by_level[t]];
+    Symbolic s = analyze_order(net, yp, tail_width, perm, gp, gi);
+    // trailing dense chain: node n-1-i has exactly i entries below its pivot
+    // (then they are all the later nodes); keep it level-aligned (its first
+    // node alone on its level) and at most chain_cap long
+    int m = 0;
+    while (m < n && m < chain_cap && s.colptr[n - m] - s.colptr[n - 1 - m] == m) m++;
+    if (m > 0 && s.lvl_ptr[s.level[n - m]] != n - m) m--;
+    if (m < 4) m = 0;
+    s.ch_m = m;
+    s.ch_c0 = n - m;
+    s.nlev_s = s.nlev - m;
+    if (m > 0) {
+        std::vector<int> pcol(s.nnzL);
+        for (int k = 0; k < n; k++)
+            for (int p = s.colptr[k]; p < s.colptr[k + 1]; p++) pcol[p] = k;
+        const int d0 = s.fd_ptr[s.ch_c0], d1 = s.fd_ptr[n];
+        s.ch_split.resize(d1 - d0);
+        for (int d = d0; d < d1; d++) {
+            int q = s.fc_ptr[d];
+            while (q < s.fc_ptr[d + 1] && pcol[s.fc_b[q]] < s.ch_c0) q++;   // pairs ascend in k
+            s.ch_split[d - d0] = q;
+        }
+    }
+    return s;
+}
+
+void partition_subtrees(Symbolic &s, int max_parts) {
+    const int n = s.n, T = s.tail_start;
+    s.n_parts = 0;
+    s.part_ptr.clear();
+    s.part_nodes.clear();
+    if (T == 0 || max_parts <= 0) return;
+    const int nw = s.lvl_ptr[T];   // nodes [0, nw) are below the tail (level order)
+    // work of a node: its destinations and update pairs; summed over subtrees
+    std::vector<long long> work(nw);
+    for (int j = 0; j < nw; j++)
+        work[j] += (s.fd_ptr[j + 1] - s.fd_ptr[j]) + (s.fc_ptr[s.fd_ptr[j + 1]] - s.fc_ptr[s.fd_ptr[j]]);
+    std::vector<int> roots;
+    for (int j = 0; j < nw; j++) {   // children precede parents
+        const int p = s.parent[j];
+        if (p >= 0 && p < nw) work[p] += work[j];
+        else roots.push_back(j);
+    }
+    std::sort(roots.begin(), roots.end(), [&](int a, int b) { return work[a] != work[b] ? work[a] > work[b] : a < b; });
+    const int np = std::min<int>(max_parts, (int)roots.size());
+    std::vector<long long> load(np, 0);
+    std::vector<int> part(n, -1);
+    for (int r : roots) {
+        int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        part[r] = g;
+        load[g] += work[r];
+    }
+    for (int j = nw - 1; j >= 0; j--)   // parents precede children going down
+        if (part[j] < 0) part[j] = part[s.parent[j]];
+    s.n_parts = np;
+    s.part_ptr.assign((size_t)np * T + 1, 0);
+    for (int j = 0; j < nw; j++) s.part_ptr[(size_t)part[j] * T + s.level[j] + 1]++;
+    for (size_t i = 0; i + 1 < s.part_ptr.size(); i++) s.part_ptr[i + 1] += s.part_ptr[i];
+    s.part_nodes.resize(nw);
+    std::vector<int> fill(s.part_ptr.begin(), s.part_ptr.end() - 1);
+    for (int j = 0; j < nw; j++) s.part_nodes[fill[(size_t)part[j] * T + s.level[j]]++] = j;
+}
+
 // ------------------------------------------------------------- host numeric
 
 namespace {
@@ -287,11 +375,78 @@ inline void mms(double *c, const double *a, const double *b) {   // c -= a*b (2x
     c[0] = c0; c[1] = c1; c[2] = c2; c[3] = c3;
 }
 
+inline void mul2(double *c, const double *a, const double *b) {   // c = a*b (2x2)
+    double c0 = a[0] * b[0] + a[1] * b[2], c1 = a[0] * b[1] + a[1] * b[3];
+    double c2 = a[2] * b[0] + a[3] * b[2], c3 = a[2] * b[1] + a[3] * b[3];
+    c[0] = c0; c[1] = c1; c[2] = c2; c[3] = c3;
+}
+
+// Trailing dense chain: Schur complement of the chain block and its
+// Gauss-Jordan inverse, stored row-major at block n + 3 nnzL (the
+// arithmetic of the device chain factorisation).
+bool host_chain_factor(const Symbolic &s, double *blk) {
+    const int n = s.n, m = s.ch_m, c0 = s.ch_c0, L0 = n, U0 = n + s.nnzL, SB = n + 3 * s.nnzL;
+    double *S = blk + 4 * (size_t)SB;
+    auto at = [&](int i, int j) { return S + 4 * ((size_t)i * m + j); };
+    const int dbase = s.fd_ptr[c0];
+    for (int j = c0; j < n; j++)
+        for (int d = s.fd_ptr[j]; d < s.fd_ptr[j + 1]; d++) {
+            const int t = d - s.fd_ptr[j], a = j - c0, qend = s.ch_split[d - dbase];
+            if (t == 0) {
+                double D[4];
+                std::copy(blk + 4 * (size_t)j, blk + 4 * (size_t)j + 4, D);
+                for (int q = s.fc_ptr[d]; q < qend; q++)
+                    mms(D, blk + 4 * (size_t)(L0 + s.fc_a[q]), blk + 4 * (size_t)(U0 + s.fc_b[q]));
+                std::copy(D, D + 4, at(a, a));
+            } else {
+                const int p = s.colptr[j] + t - 1, i = a + t;
+                double Lb[4], Ub[4];
+                std::copy(blk + 4 * (size_t)(L0 + p), blk + 4 * (size_t)(L0 + p) + 4, Lb);
+                std::copy(blk + 4 * (size_t)(U0 + p), blk + 4 * (size_t)(U0 + p) + 4, Ub);
+                for (int q = s.fc_ptr[d]; q < qend; q++) {
+                    const int pik = s.fc_a[q], pjk = s.fc_b[q];
+                    mms(Lb, blk + 4 * (size_t)(L0 + pik), blk + 4 * (size_t)(U0 + pjk));
+                    mms(Ub, blk + 4 * (size_t)(L0 + pjk), blk + 4 * (size_t)(U0 + pik));
+                }
+                std::copy(Lb, Lb + 4, at(i, a));
+                std::copy(Ub, Ub + 4, at(a, i));
+            }
+        }
+    std::vector<double> row(4 * (size_t)m), col(4 * (size_t)m);
+    for (int k = 0; k < m; k++) {
+        for (int t = 0; t < m; t++) {
+            std::copy(at(k, t), at(k, t) + 4, &row[4 * t]);
+            std::copy(at(t, k), at(t, k) + 4, &col[4 * t]);
+        }
+        const double *pv = &row[4 * k];
+        const double det = pv[0] * pv[3] - pv[1] * pv[2];
+        if (!(det != 0.0) || !std::isfinite(det)) return false;
+        const double P[4] = {pv[3] / det, -pv[1] / det, -pv[2] / det, pv[0] / det};
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                double *sij = at(i, j);
+                if (i == k) {
+                    if (j == k) std::copy(P, P + 4, sij);
+                    else mul2(sij, P, &row[4 * j]);
+                } else if (j == k) {
+                    mul2(sij, &col[4 * i], P);
+                    for (int c = 0; c < 4; c++) sij[c] = -sij[c];
+                } else {
+                    double u[4];
+                    mul2(u, P, &row[4 * j]);
+                    mms(sij, &col[4 * i], u);
+                }
+            }
+    }
+    return true;
+}
+
 }  // namespace
 
 bool host_block_factor(const Symbolic &s, double *blk) {
     const int n = s.n, L0 = n, U0 = n + s.nnzL;
-    for (int j = 0; j < n; j++) {
+    const int nf = s.ch_m > 0 ? s.ch_c0 : n;   // nodes factored one by one
+    for (int j = 0; j < nf; j++) {
         double dinv[4];
         for (int d = s.fd_ptr[j]; d < s.fd_ptr[j + 1]; d++) {
             int t = d - s.fd_ptr[j];
@@ -320,23 +475,41 @@ bool host_block_factor(const Symbolic &s, double *blk) {
             }
         }
     }
+    if (s.ch_m > 0) return host_chain_factor(s, blk);
     return true;
 }
 
 void host_block_solve(const Symbolic &s, const double *blk, double *x) {
     const int n = s.n, L0 = n, U0 = n + s.nnzL;
+    const int nf = s.ch_m > 0 ? s.ch_c0 : n;   // chain rows take only x_C - L_CN y_N here
     for (int j = 0; j < n; j++) {
         double y0 = x[2 * j], y1 = x[2 * j + 1];
         for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) {
+            const int k = s.rs_k[q];
+            if (k >= nf) break;                  // row entries ascend in k
             const double *L = blk + 4 * (size_t)(L0 + s.rs_p[q]);
-            int k = s.rs_k[q];
             y0 -= L[0] * x[2 * k] + L[1] * x[2 * k + 1];
             y1 -= L[2] * x[2 * k] + L[3] * x[2 * k + 1];
         }
         x[2 * j] = y0;
         x[2 * j + 1] = y1;
     }
-    for (int j = n - 1; j >= 0; j--) {
+    if (s.ch_m > 0) {   // x_C = S^-1 (x_C - L_CN y_N)
+        const int m = s.ch_m, c0 = s.ch_c0;
+        const double *S = blk + 4 * (size_t)(n + 3 * s.nnzL);
+        std::vector<double> r(x + 2 * (size_t)c0, x + 2 * (size_t)n);
+        for (int a = 0; a < m; a++) {
+            double a0 = 0, a1 = 0;
+            for (int b = 0; b < m; b++) {
+                const double *Sab = S + 4 * ((size_t)a * m + b);
+                a0 += Sab[0] * r[2 * b] + Sab[1] * r[2 * b + 1];
+                a1 += Sab[2] * r[2 * b] + Sab[3] * r[2 * b + 1];
+            }
+            x[2 * (c0 + a)] = a0;
+            x[2 * (c0 + a) + 1] = a1;
+        }
+    }
+    for (int j = nf - 1; j >= 0; j--) {
         double y0 = x[2 * j], y1 = x[2 * j + 1];
         for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) {
             const double *U = blk + 4 * (size_t)(U0 + p);

@@ -76,6 +76,11 @@ typedef struct pf_options {
     int precision;          /* pf_precision                                           */
     int refine_steps;       /* FP64 refinement steps after an FP32 solve (default 2)  */
     int use_graph;          /* 1: one captured hipGraph per NR iteration, no per-iteration host sync */
+    int lu_blocked;         /* sparse LU schedule.  1 (default): the subtrees below the narrow
+                               top of the elimination tree in one launch (a workgroup per group
+                               of subtrees) and its trailing dense chain as one dense block
+                               (Gauss-Jordan inverse of its Schur complement); 0: one launch
+                               per elimination-tree level, node by node                        */
 } pf_options;
 
 const char *pf_status_str(pf_status s);
@@ -184,6 +189,18 @@ pf_status pf_probe_get(const pf_symbolic_probe *p, int *perm, int *slot_e, int *
 /* elimination-tree levels: lvl_ptr [levels+1] (nodes of level l are the
  * positions lvl_ptr[l] .. lvl_ptr[l+1]-1), column pointers of L [n+1]      */
 pf_status pf_probe_levels(const pf_symbolic_probe *p, int *lvl_ptr, int *colptr);
+/* trailing dense chain of the elimination tree (nodes c0 .. n-1, one per
+ * level, full columns) that the solver factors as one dense block; m = 0:
+ * none (shorter than 4 nodes)                                               */
+pf_status pf_probe_chain(const pf_symbolic_probe *p, int *m, int *c0);
+/* groups of whole subtrees below the narrow top of the tree, one workgroup
+ * each (as the solver builds them for 256 CUs): group g owns the nodes
+ * part_nodes[part_ptr[g*part_levels + l] .. part_ptr[g*part_levels + l + 1])
+ * of level l; part_ptr has n_parts*part_levels + 1 entries, part_nodes one
+ * per node below the top; parent: elimination tree (n, -1 at the root).
+ * Any pointer may be NULL.                                                  */
+pf_status pf_probe_parts(const pf_symbolic_probe *p, int *n_parts, int *part_levels, int *part_ptr,
+                         int *part_nodes, int *parent);
 /* blocks: nslot 2x2 blocks (row-major), x: 2n right-hand side in node order */
 pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x);
 
@@ -204,6 +221,25 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of,
 pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms);
 pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations,
                        int *converged, double *max_mismatch);
+/* Autotuner (no counterpart in the reference): for every topology of the
+ * current scenario set, times one launch of its persistent kernel for each
+ * candidate launch shape (working set fully in LDS / step vectors and level
+ * metadata in LDS / all global; workgroup size; resident workgroups per CU),
+ * keeps the fastest (minimum over `reps` timed launches after one warm-up)
+ * and appends it to the text cache at cache_path (may be NULL: no cache).
+ * A shape found in the cache for the same device, topology sizes, options and
+ * scenario count is applied without measuring.  Outputs of the batch are
+ * overwritten; call pf_batch_solve again afterwards.                        */
+pf_status pf_batch_autotune(pf_batch *b, const char *cache_path, int reps, int *n_measured);
+/* fix the launch shape of network `net` (after pf_batch_set_scenarios):
+ * mode 2 needs the working set to fit in LDS, 1 the step vectors and level
+ * metadata, 0 always fits; block a multiple of 64 up to 1024; per_cu <= 0:
+ * as many workgroups per CU as fit.  PF_INVALID_ARG when it cannot run.     */
+pf_status pf_batch_set_shape(pf_batch *b, int net, int mode, int block, int per_cu);
+/* launch shape in use for network `net`: kernel mode (2 LDS, 1 partial LDS,
+ * 0 global), workgroup size, workgroups, dynamic LDS bytes                   */
+pf_status pf_batch_get_shape(pf_batch *b, int net, int *mode, int *block, int *grid,
+                             long long *lds_bytes);
 
 /* ------------------------------------------------------------ warm start */
 /* MLP: injections + setpoints -> |V|, theta.  BF16 weights/activations,

@@ -127,3 +127,96 @@ def test_batch_warm_start_and_fallback(pf, oracle, synth):
     assert cv.all()
     for s in range(8):
         assert np.abs(V[118 * s:118 * (s + 1)] - ref[s]["V"]).max() <= 1e-6
+
+
+def test_batch_autotune(pf, synth, cases, tmp_path):
+    """The autotuner changes only launch shapes: every candidate does the same
+    arithmetic in the same order (per-row and per-node lane groups do not
+    depend on the workgroup size or on where the working set lives), so the
+    tuned answers equal the default-shape answers bit for bit; a second batch
+    with the same cache applies the cached shapes without measuring."""
+    p118 = os.path.join(GOLDEN, "c118.cdf")
+    n118, n944 = pf.Network(p118), pf.Network(cases.path("c944"))
+    A = _scenarios(synth, n118, 64, 3000000)
+    B = _scenarios(synth, n944, 8, 3200000)
+    net_of = np.array([1] * 8 + [0] * 64, np.int32)
+    P = np.concatenate([B[0].ravel(), A[0].ravel()])
+    Q = np.concatenate([B[1].ravel(), A[1].ravel()])
+    L = np.concatenate([B[2].ravel(), A[2].ravel()])
+    opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, precision=pf.MIXED)
+    cache = str(tmp_path / "tune.txt")
+
+    def run(tune):
+        b = pf.Batch([n118, n944], opts)
+        b.set_scenarios(net_of, P, Q, L)
+        n = b.autotune(cache, reps=1) if tune else 0
+        b.solve()
+        return b, n, b.results()
+
+    _say("autotune default shapes")
+    b0, _, r0 = run(False)
+    _say("default", b0.shape(0), b0.shape(1))
+    _say("autotune measuring")
+    b1, n1, r1 = run(True)
+    _say("tuned", b1.shape(0), b1.shape(1), "candidates", n1)
+    assert n1 >= 2
+    for x, y in zip(r0, r1):
+        assert np.array_equal(x, y)
+    _say("autotune from cache")
+    b2, n2, r2 = run(True)
+    assert n2 == 0
+    assert b2.shape(0) == b1.shape(0) and b2.shape(1) == b1.shape(1)
+    for x, y in zip(r0, r2):
+        assert np.array_equal(x, y)
+    assert r0[3].all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_every_shape(pf, oracle, synth, cases, precision):
+    """Every launch shape (working set in LDS / step vectors and level data in
+    LDS / all global; workgroup size) and both LU modes (the dense separator
+    block on or off) solve each scenario like the oracle.  All results are
+    printed before the first assertion."""
+    p118, p944 = os.path.join(GOLDEN, "c118.cdf"), cases.path("c944")
+    n118, n944 = pf.Network(p118), pf.Network(p944)
+    o118, o944 = oracle.Case(p118), oracle.Case(p944)
+    A = _scenarios(synth, n118, 6, 3300000)
+    B = _scenarios(synth, n944, 2, 3400000)
+    ref = [o118.solve(tol=1e-8, lim=True, Pliq=A[0][s], Qliq=A[1][s], Qload=A[2][s], reuse_symbolic=True)
+           for s in range(6)]
+    ref += [o944.solve(tol=1e-8, lim=True, Pliq=B[0][s], Qliq=B[1][s], Qload=B[2][s], reuse_symbolic=True)
+            for s in range(2)]
+    net_of = np.array([0] * 6 + [1] * 2, np.int32)
+    P = np.concatenate([A[0].ravel(), B[0].ravel()])
+    Q = np.concatenate([A[1].ravel(), B[1].ravel()])
+    L = np.concatenate([A[2].ravel(), B[2].ravel()])
+    rows = []
+    for lu_blocked in (0, 1):
+        opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, lu_blocked=lu_blocked,
+                          precision=pf.MIXED if precision == "mixed" else pf.FP64)
+        b = pf.Batch([n118, n944], opts)
+        b.set_scenarios(net_of, P, Q, L)
+        for mode in (2, 1, 0):
+            for block in (256, 1024):
+                try:
+                    b.set_shape(0, mode, block)
+                    b.set_shape(1, mode if mode < 2 else 0, block)
+                except pf.PfError:
+                    continue
+                b.solve()
+                V, th, it, cv, mm = b.results()
+                dv = dt = 0.0
+                off = 0
+                for s, r in enumerate(ref):
+                    nb = (n118, n944)[net_of[s]].nB
+                    if cv[s] and r["converged"]:
+                        dv = max(dv, float(np.abs(V[off:off + nb] - r["V"]).max()))
+                        dt = max(dt, float(np.abs(th[off:off + nb] - r["theta"]).max()))
+                    off += nb
+                row = (lu_blocked, b.shape(0)["mode"], b.shape(1)["mode"], block, int(cv.sum()), dv, dt,
+                       it.tolist(), [r["iterations"] for r in ref])
+                _say("lu_blocked %d modes %d/%d block %d converged %d dV %.3g dth %.3g iters %s oracle %s" % row)
+                rows.append(row)
+    assert rows
+    for row in rows:
+        assert row[4] == len(ref) and row[5] <= 1e-6 and row[6] <= 1e-6, row

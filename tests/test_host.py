@@ -134,8 +134,11 @@ def _dense_from_slots(net, pr, blocks):
     return A
 
 
-@pytest.mark.parametrize("which", ["ieee14", "ieee118", "c118", "c944"])
+@pytest.mark.parametrize("which", ["ieee14", "ieee118", "c118", "c944", "c1888"])
 def test_symbolic_program_solves(pf, cases, which):
+    """Factor + solve over the symbolic program (nodes numbered level by level,
+    the trailing dense chain as one Gauss-Jordan-inverted Schur complement)
+    against numpy's dense solve of the same matrix."""
     path = os.path.join(IEEE, which + ".cdf") if which.startswith("ieee") else cases.path(which)
     net = pf.Network(path)
     pr = pf.SymbolicProbe(net)
@@ -154,6 +157,31 @@ def test_symbolic_program_solves(pf, cases, which):
     x = pr.factor_solve(blocks, b)
     xr = np.linalg.solve(_dense_from_slots(net, pr, blocks), b)
     assert np.abs(x - xr).max() <= 1e-12 * np.abs(xr).max()
+    # levels are contiguous node ranges, and the chain is the last levels
+    lp = pr.lvl_ptr
+    assert lp[0] == 0 and lp[-1] == pr.n and np.all(np.diff(lp) > 0)
+    # subtree groups: every node below the top exactly once, on its own level,
+    # and each group closed under the elimination tree below the top
+    npart, nl, ptr, nodes, parent = pr.parts()
+    nw = lp[nl]
+    assert sorted(nodes.tolist()) == list(range(nw))
+    group = np.full(pr.n, -1)
+    for g in range(npart):
+        for l in range(nl):
+            seg = nodes[ptr[g * nl + l]:ptr[g * nl + l + 1]]
+            assert np.all((seg >= lp[l]) & (seg < lp[l + 1]))
+            group[seg] = g
+    for j in range(nw):
+        if 0 <= parent[j] < nw:
+            assert group[parent[j]] == group[j]
+    m, c0 = pr.chain
+    if which.startswith("c"):   # the generated meshed networks end in a dense separator chain
+        assert m >= 4
+    if m:
+        assert c0 == pr.n - m
+        assert list(lp[-m - 1:]) == list(range(c0, pr.n + 1))          # one node per level
+        cnt = np.diff(pr.colptr)
+        assert list(cnt[c0:]) == list(range(m - 1, -1, -1))             # full columns
 
 
 def test_splitmix64_known_answers(synth):

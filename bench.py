@@ -106,8 +106,13 @@ def warm_start_leg(pf, synth, net, opts, n_train, n_eval, epochs, seed_train, se
     vset = np.where(is_pv, net.V, 0.0)
     vset[net.slack] = net.V[net.slack]
 
+    n_in, n_out = 3 * net.nB, 2 * net.nB
+    n_in8, n_out8 = -(-n_in // 8) * 8, -(-n_out // 8) * 8   # the MLP takes multiples of 8
+
     def feats(P, Q):
-        return np.concatenate([P, Q, np.repeat(vset[None, :], len(P), 0)], 1).astype(np.float32)
+        X = np.zeros((len(P), n_in8), np.float32)
+        X[:, :n_in] = np.concatenate([P, Q, np.repeat(vset[None, :], len(P), 0)], 1)
+        return X
 
     def solve(P, Q, L, V0=None, T0=None, reps=1):
         b = pf.Batch([net], opts)
@@ -125,10 +130,12 @@ def warm_start_leg(pf, synth, net, opts, n_train, n_eval, epochs, seed_train, se
     V, th, it, cv, ms_data = solve(P, Q, L)
     log(f"warm start {net.nB}-bus: training, {epochs} epochs")
     ok = cv.astype(bool)
-    X, Y = feats(P, Q)[ok], np.concatenate([V, th], 1).astype(np.float32)[ok]
+    X = feats(P, Q)[ok]
+    Y = np.zeros((int(ok.sum()), n_out8), np.float32)
+    Y[:, :n_out] = np.concatenate([V, th], 1)[ok]
     t_data = time.perf_counter() - t
-    w = pf.WarmStart(3 * net.nB, 2 * net.nB, 512, 3, seed=11)
-    w.set_norm(X.mean(0), X.std(0) + 1e-6, Y.mean(0), Y.std(0) + 1e-9)
+    w = pf.WarmStart(n_in8, n_out8, 512, 3, seed=11)
+    w.set_norm(X.mean(0), X.std(0) + 1e-6, Y.mean(0), Y.std(0) + 1e-9)   # padding: mean 0, std ~1e-9
     t = time.perf_counter()
     losses = w.fit(X, Y, epochs=epochs, batch=256, lr=1e-3, weight_decay=1e-5)
     t_fit = time.perf_counter() - t
@@ -142,11 +149,11 @@ def warm_start_leg(pf, synth, net, opts, n_train, n_eval, epochs, seed_train, se
     t_pred = time.perf_counter() - t
     Vf, thf, itf, cvf, msf = solve(Pt, Qt, Lt, reps=3)
     Vw, thw, itw, cvw, msw = solve(Pt, Qt, Lt, Yt[:, :net.nB].astype(np.float64),
-                                   Yt[:, net.nB:].astype(np.float64), reps=3)
+                                   Yt[:, net.nB:n_out].astype(np.float64), reps=3)
     both = cvf.astype(bool) & cvw.astype(bool)
     return {"buses": net.nB, "train_scenarios": int(n_train), "train_converged": int(cv.sum()),
             "eval_scenarios": int(n_eval), "epochs": epochs,
-            "mlp": f"{3 * net.nB}->512x3 GELU->{2 * net.nB}, bf16 weights, fp32 accumulate",
+            "mlp": f"{n_in8}->512x3 GELU->{n_out8}, bf16 weights, fp32 accumulate",
             "train_data_s": round(t_data, 2), "fit_s": round(t_fit, 2),
             "loss_first_last": [round(float(losses[0]), 5), round(float(losses[-1]), 6)],
             "iterations_flat_mean": round(float(itf.mean()), 3),

@@ -91,6 +91,8 @@ def lib():
         L.pf_solver_lu_info.argtypes = [P, P, P, P, P]
         L.pf_stage_ybus.argtypes = [P, P, P, P, P]
         L.pf_stage_pq.argtypes = [P, P, P, P, P]
+        L.pf_stage_flows.argtypes = [P, P, P, P, P, P, P]
+        L.pf_solver_flows.argtypes = [P, P, P, P, P]
         L.pf_stage_mismatch.argtypes = [P, P, P, P, P, P, P]
         L.pf_stage_jacobian.argtypes = [P, P, P, P, P, P, P, P, C.c_int, P]
         L.pf_stage_lu_solve.argtypes = [P, P, P, P, P, P]
@@ -304,6 +306,18 @@ class Solver:
         Q = np.zeros(self.net.nB)
         _chk(lib().pf_stage_pq(self.h, _p(V), _p(th), _p(P), _p(Q)), "pf_stage_pq")
         return P, Q
+
+    def flows(self, V=None, theta=None):
+        """Branch flows as the reference reports them, (Pdp, Qdp, Ppd, Qpd)
+        [nL]: at the given state, or at the solver's current one."""
+        out = [np.zeros(self.net.nL) for _ in range(4)]
+        if V is None:
+            _chk(lib().pf_solver_flows(self.h, *[_p(a) for a in out]), "pf_solver_flows")
+        else:
+            V = np.ascontiguousarray(V, np.float64)
+            th = np.ascontiguousarray(theta, np.float64)
+            _chk(lib().pf_stage_flows(self.h, _p(V), _p(th), *[_p(a) for a in out]), "pf_stage_flows")
+        return tuple(out)
 
     def _lq(self, limQ):
         if limQ is None:

@@ -73,6 +73,12 @@ pf_status pf_network_get_bus(const pf_network *net, int field, double *out) {
     return PF_OK;
 }
 
+pf_status pf_network_get_ids(const pf_network *net, int *id) {
+    if (!net || !id) return PF_INVALID_ARG;
+    std::memcpy(id, net->id.data(), sizeof(int) * net->nB);
+    return PF_OK;
+}
+
 pf_status pf_network_get_lists(const pf_network *net, int *pv, int *pq, double *qmin, double *qmax,
                                double *vset) {
     if (!net) return PF_INVALID_ARG;

@@ -145,7 +145,7 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
                     det = (T)1;
                 }
                 di = B4<T>{a.d / det, -a.b / det, -a.c / det, a.a / det};
-                store_b4(blk, j, di);
+                store_b4(blk, Y.di0 + j, di);
             }
             dinv.a = __shfl(di.a, 0, GL);
             dinv.b = __shfl(di.b, 0, GL);
@@ -226,7 +226,7 @@ __device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
         const int s = Y.lvl_ptr[l];
         if (grp < Y.lvl_ptr[l + 1] - s) {
             const int j = s + grp, p = Y.colptr[j] + lane;
-            if (lane == 0) D = load_b4(blk, j);
+            if (lane == 0) D = load_b4(blk, Y.di0 + j);
             if (p < Y.colptr[j + 1]) {
                 r = Y.rowind[p];
                 U = load_b4(blk, U0 + p);
@@ -266,7 +266,7 @@ __device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
             }
             if (lane == 0) {
                 const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
-                const B4<T> D = (i == grp) ? pD : load_b4(blk, j);
+                const B4<T> D = (i == grp) ? pD : load_b4(blk, Y.di0 + j);
                 x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
                 x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
             }
@@ -504,17 +504,21 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
         __syncthreads();
         // factor, level by level
         // wide levels: a lane group per node; narrow levels: the whole
-        // workgroup on the level's destinations (coop_factor_level)
+        // workgroup on the level's destinations (coop_factor_range)
         {
             const int grp = tid / GL, lane = tid % GL, ng = (int)blockDim.x / GL;
-            for (int l = 0; l < Y.nlev_s; l++) {
+            for (int l = 0; l < Y.nlev_s;) {
                 const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
                 if (c >= kWideLevel) {
                     for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, (s + i), lane, &s_sing);
-                } else {
-                    coop_factor_level<T, GL>(Y, w.blk, l, &s_sing);
+                    __syncthreads();
+                    l++;
+                } else {   // a run of narrow levels
+                    int l1 = l + 1;
+                    while (l1 < Y.nlev_s && Y.lvl_ptr[l1 + 1] - Y.lvl_ptr[l1] < kWideLevel) l1++;
+                    coop_factor_range<T, GL>(Y, w.blk, l, l1, &s_sing);
+                    l = l1;
                 }
-                __syncthreads();
             }
             if (Y.ch_m > 0) {
                 chain_factor_mem<T, GL, kBatchChain>(Y, w.blk, &s_sing);
@@ -785,7 +789,7 @@ pf_status pf_batch::add_net(const pf_network &src) {
     BCHK(upload((double **)&N.qmax, net.Qmax));
     BCHK(upload((double **)&N.vset, net.Vset));
     Y.n = n; Y.nnzL = sym.nnzL; Y.nslot = sym.nslot(); Y.nlev = sym.nlev;
-    Y.ch_m = sym.ch_m; Y.ch_c0 = sym.ch_c0; Y.nlev_s = sym.nlev_s;
+    Y.ch_m = sym.ch_m; Y.ch_c0 = sym.ch_c0; Y.nlev_s = sym.nlev_s; Y.di0 = sym.di0();
     BCHK(upload((int **)&Y.ch_split, sym.ch_split));
     BCHK(upload((int **)&Y.colptr, sym.colptr));
     BCHK(upload((int **)&Y.rowind, sym.rowind));
@@ -1070,6 +1074,8 @@ pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms) {
 // keep the fastest, and remember it in a text cache keyed by the device, the
 // topology's sizes, the numeric options and the scenario count, so later
 // runs apply it without measuring.
+}  // extern "C"
+
 namespace {
 
 std::string tune_key(const pf_batch &b, const BatchNet &nb, const char *arch) {
@@ -1109,6 +1115,8 @@ void cache_store(const char *path, const std::string &key, int mode, int block, 
 }
 
 }  // namespace
+
+extern "C" {
 
 pf_status pf_batch_autotune(pf_batch *b, const char *cache_path, int reps, int *n_measured) {
     if (!b || !b->n_scen || reps < 0) return PF_INVALID_ARG;

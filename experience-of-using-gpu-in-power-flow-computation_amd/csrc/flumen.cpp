@@ -1,7 +1,9 @@
 // flumen: command-line drop-in for the reference executables' solve path.
 // Reads data.txt from the working directory (CPU/main.cpp:28-36,
 // GPU/main.cu:25-33), the CDF/CDFX case it names, runs the NR power flow on
-// GPU 0 with the selected metodo, and prints the solved state.
+// GPU 0 with the selected metodo, and prints the solved state.  With
+// laconic_mode OFF and output_ans ON it prints the reference's flow report
+// (impressao2, CPU/impressao.cpp:3-36, called from CPU/main.cpp:122-125).
 //
 //   flumen [--cpu-layout] [data.txt]
 //
@@ -12,6 +14,40 @@
 #include <vector>
 
 #include "../../include/pf.h"
+
+// The reference's flow report, line for line: per bus V, theta (rad),
+// injections, generation (injection + load), then the flows of the branches
+// at that bus in branch order, the first on the bus line.
+static void report(const pf_network *net, const pf_network_info &info, pf_solver *s,
+                   const std::vector<double> &V, const std::vector<double> &th) {
+    const int nB = info.nB, nL = info.nL;
+    std::vector<int> id(nB), from(nL), to(nL);
+    std::vector<double> P(nB), Q(nB), Pload(nB), Qload(nB), Pdp(nL), Qdp(nL), Ppd(nL), Qpd(nL);
+    pf_network_get_ids(net, id.data());
+    pf_network_get_bus(net, PF_BUS_PLOAD, Pload.data());
+    pf_network_get_bus(net, PF_BUS_QLOAD, Qload.data());
+    pf_network_get_branches(net, from.data(), to.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (pf_stage_pq(s, V.data(), th.data(), P.data(), Q.data()) != PF_OK ||
+        pf_stage_flows(s, V.data(), th.data(), Pdp.data(), Qdp.data(), Ppd.data(), Qpd.data()) != PF_OK) {
+        std::fprintf(stderr, "flumen: flows: %s\n", pf_last_error());
+        return;
+    }
+    std::printf("\n\n                                 FLUXOS DE POTENCIA\n");
+    std::printf("        BUS      V         ANG        Pli        Qli        Pge        Qge      TO        PIJ          QIJ\n");
+    for (int i = 0; i < nB; i++) {
+        std::printf("      %4d%11.5f%11.5f%11.5f%11.5f%11.5f%11.5f  ", id[i], V[i], th[i], P[i], Q[i], P[i] + Pload[i],
+                    Q[i] + Qload[i]);
+        bool first = true;
+        for (int j = 0; j < nL; j++) {
+            const bool fwd = from[j] == i, bwd = !fwd && to[j] == i;
+            if (!fwd && !bwd) continue;
+            if (!first) std::printf("%78s", "");
+            std::printf("%4d  %11.5f  %11.5f\n", id[fwd ? to[j] : from[j]], fwd ? Pdp[j] : Ppd[j],
+                        fwd ? Qdp[j] : Qpd[j]);
+            first = false;
+        }
+    }
+}
 
 int main(int argc, char **argv) {
     int cpu_layout = 0;
@@ -47,11 +83,7 @@ int main(int argc, char **argv) {
     st = pf_solve(s, &r);
     std::vector<double> V(info.nB), th(info.nB);
     pf_solver_get_state(s, V.data(), th.data(), nullptr);
-    if (!opt.laconic_mode && opt.output_ans) {
-        std::printf("\n  bus        V (pu)    theta (deg)\n");
-        for (int i = 0; i < info.nB; i++)
-            std::printf("%5d  %12.6f  %12.6f\n", i + 1, V[i], th[i] * 180.0 / 3.14159265358979323846);
-    }
+    if (!opt.laconic_mode && opt.output_ans) report(net, info, s, V, th);
     std::printf("iterations = %d; max|g| = %e; PV<->PQ switches = %d; NR loop = %.3f ms\n", r.iterations,
                 r.max_mismatch, r.n_switch_events, r.solve_ms);
     if (st != PF_OK) std::printf("O processo iterativo nao convergiu! (%s)\n", pf_status_str(st));

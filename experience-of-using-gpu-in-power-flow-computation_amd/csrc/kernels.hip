@@ -306,7 +306,7 @@ __device__ __forceinline__ void factor_node(const DevSym &Y, T *blk, int j, int 
                     det = (T)1;
                 }
                 di = B4<T>{a.d / det, -a.b / det, -a.c / det, a.a / det};
-                store_b4(blk, j, di);
+                store_b4(blk, Y.di0 + j, di);
             }
             dinv.a = __shfl(di.a, 0);
             dinv.b = __shfl(di.b, 0);
@@ -333,10 +333,7 @@ __global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    for (int l = lv0; l < lv1; l++) {
-        coop_factor_level<T, 16>(Y, blk, l, &ctl->singular);
-        __syncthreads();
-    }
+    coop_factor_range<T, 16>(Y, blk, lv0, lv1, &ctl->singular);
     if (Y.ch_m > 0) chain_factor<T, 16, kChainRegs, kChainMax>(Y, blk, &ctl->singular);
 }
 
@@ -381,7 +378,7 @@ __device__ __forceinline__ void bwd_node(const DevSym &Y, const T *blk, double *
     }
     if (sl == 0) {
         const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
-        B4<T> D = load_b4(blk, j);
+        B4<T> D = load_b4(blk, Y.di0 + j);
         x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
         x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
     }
@@ -514,7 +511,7 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
         const int s = Y.lvl_ptr[l];
         if (w < Y.lvl_ptr[l + 1] - s) {
             const int j = s + w, p = Y.colptr[j] + lane;
-            if (lane == 0) D = load_b4(blk, j);
+            if (lane == 0) D = load_b4(blk, Y.di0 + j);
             if (p < Y.colptr[j + 1]) {
                 r = Y.rowind[p];
                 U = load_b4(blk, U0 + p);
@@ -551,7 +548,7 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
             a1 = wave_sum(a1);
             if (lane == 0) {
                 const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
-                const B4<T> D = (idx == w) ? pD : load_b4(blk, j);
+                const B4<T> D = (idx == w) ? pD : load_b4(blk, Y.di0 + j);
                 x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
                 x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
             }
@@ -561,6 +558,40 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
         pU = nU;
         pD = nD;
     }
+}
+
+// ------------------------------------------------------- branch flows
+// Flows as the reference reports them (SpCalcFlux,
+// CPU/esparso/potencia/spCalcFluxPQ.cpp:3-36): for the direction k -> m,
+// a = phif(k,m) + th_k - th_m and G + jB = Ybus(k,m),
+//   P = V_k^2 G - V_k V_m G cos a - V_k V_m B sin a
+//   Q = -V_k^2 (B + bshf(k,m)) + V_k V_m B cos a - V_k V_m G sin a
+// evaluated left to right without contraction.  One thread per branch;
+// out = [Pdp | Qdp | Ppd | Qpd], nL each.
+__device__ __forceinline__ void flow_dir(const DevNet &N, const DevState &S, int k, int m, int slot,
+                                         double phi, double bsh, double &P, double &Q) {
+#pragma clang fp contract(off)
+    const double2 y = N.y[slot];
+    const double G = y.x, B = y.y;
+    const double a = phi + S.th[k] - S.th[m];
+    double sn, cs;
+    sincos(a, &sn, &cs);
+    const double vkk = S.V[k] * S.V[k], vkm = S.V[k] * S.V[m];
+    P = vkk * G - vkm * G * cs - vkm * B * sn;
+    Q = -vkk * (B + bsh) + vkm * B * cs - vkm * G * sn;
+}
+
+__global__ void k_flows(DevNet N, DevState S, const FlowBranch *br, int nL, double *out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nL) return;
+    const FlowBranch b = br[l];
+    double P, Q;
+    flow_dir(N, S, b.from, b.to, b.slot_ft, b.phi_ft, b.bsh_ft, P, Q);
+    out[l] = P;
+    out[nL + l] = Q;
+    flow_dir(N, S, b.to, b.from, b.slot_tf, b.phi_tf, b.bsh_tf, P, Q);
+    out[2 * nL + l] = P;
+    out[3 * nL + l] = Q;
 }
 
 // --------------------------------------------------------------- vectors

@@ -63,6 +63,13 @@ struct DevState {
 constexpr int kChainRegs = 10;
 static_assert(kChainMax * kChainMax <= 1024 * kChainRegs, "dense chain does not fit the registers");
 
+// Per-branch constants of the reported flows: the Ybus slots of (from, to)
+// and (to, from), and the reference's phif / bshf lookups in both directions.
+struct FlowBranch {
+    int from, to, slot_ft, slot_tf;
+    double phi_ft, phi_tf, bsh_ft, bsh_tf;
+};
+
 // Symbolic factor program on the device.
 struct DevSym {
     int n, nnzL, nslot, nlev;
@@ -74,6 +81,7 @@ struct DevSym {
     // trailing dense chain [ch_c0, n) on levels [nlev_s, nlev); ch_m = 0: none
     int ch_m, ch_c0, nlev_s;
     const int *ch_split;
+    int di0;                                // block offset of the inverse pivots
     // groups of whole subtrees below the tail, one workgroup each
     int n_parts, part_levels;
     const int *part_ptr, *part_nodes;

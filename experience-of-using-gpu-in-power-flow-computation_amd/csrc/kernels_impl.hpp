@@ -65,67 +65,96 @@ __device__ __forceinline__ B4<T> group_sum(B4<T> v) {
     return v;
 }
 
-// One elimination-tree level factored by a whole workgroup: groups of G
-// lanes take the level's destination blocks (every node's pivot and column
-// entries), the lanes of a group split that destination's update pairs and
-// reduce them in a fixed order; then the pivots are inverted and the column
-// scaled.  Three barriers per level whatever its width, so the narrow top of
-// the tree (one node per level) keeps every lane of the workgroup busy.
-// The caller synchronises after the call.
+// Narrow elimination-tree levels [l0, l1) factored by a whole workgroup:
+// groups of G lanes take a level's destination blocks (every node's pivot and
+// column entries), the lanes of a group split that destination's update pairs
+// and reduce them in a fixed order.  After a barrier the same group finishes
+// its destination: the pivot's inverse goes to the inverse-pivot region (the
+// updated pivot itself stays in place, so the column groups can invert it
+// too) and a column block is scaled by it, in both L copies.  Two barriers
+// per level whatever its width, so the narrow top of the tree (one node per
+// level) keeps every lane busy.  The indices of a group's first destination
+// and of each lane's first pair on the next level do not depend on values and
+// are loaded one level ahead.
 template <typename T, int G>
-__device__ void coop_factor_level(const DevSym &Y, T *blk, int l, int *bad) {
-    const int L0 = Y.n, U0 = Y.n + Y.nnzL;
-    const int e0 = Y.ld_ptr[l], ne = Y.ld_ptr[l + 1] - e0;
+__device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *bad) {
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int grp = (int)threadIdx.x / G, lane = (int)threadIdx.x % G, ng = (int)blockDim.x / G;
-    for (int e = grp; e < ne; e += ng) {
-        const int j = Y.ld_node[e0 + e], d = Y.ld_d[e0 + e];
-        const int t = d - Y.fd_ptr[j];
-        B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
-        if (t == 0) {
-            for (int q = Y.fc_ptr[d] + lane; q < Y.fc_ptr[d + 1]; q += G)
-                mma2(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
-        } else {
-            for (int q = Y.fc_ptr[d] + lane; q < Y.fc_ptr[d + 1]; q += G) {
-                const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
-                mma2(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
-                mma2(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
-            }
+    struct Dest {
+        int j, d, t, q, qe, pa, pb;
+    };
+    auto dest = [&](int e0, int e) {
+        Dest r;
+        r.j = Y.ld_node[e0 + e];
+        r.d = Y.ld_d[e0 + e];
+        r.t = r.d - Y.fd_ptr[r.j];
+        r.q = Y.fc_ptr[r.d] + lane;
+        r.qe = Y.fc_ptr[r.d + 1];
+        r.pa = r.pb = 0;
+        if (r.q < r.qe) {
+            r.pa = Y.fc_a[r.q];
+            r.pb = Y.fc_b[r.q];
         }
-        a = group_sum<T, G>(a);
-        u = group_sum<T, G>(u);
-        if (lane == 0) {
-            if (t == 0) {
-                B4<T> D = load_b4(blk, j);
-                store_b4(blk, j, B4<T>{D.a - a.a, D.b - a.b, D.c - a.c, D.d - a.d});
+        return r;
+    };
+    Dest cur{-1, -1, 0, 0, 0, 0, 0};
+    if (l0 < l1 && grp < Y.ld_ptr[l0 + 1] - Y.ld_ptr[l0]) cur = dest(Y.ld_ptr[l0], grp);
+    for (int l = l0; l < l1; l++) {
+        const int e0 = Y.ld_ptr[l], ne = Y.ld_ptr[l + 1] - e0;
+        Dest nxt{-1, -1, 0, 0, 0, 0, 0};
+        if (l + 1 < l1 && grp < Y.ld_ptr[l + 2] - Y.ld_ptr[l + 1]) nxt = dest(Y.ld_ptr[l + 1], grp);
+        for (int e = grp; e < ne; e += ng) {
+            const Dest D = (e == grp) ? cur : dest(e0, e);
+            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+            if (D.q < D.qe) {
+                mma2(a, load_b4(blk, L0 + D.pa), load_b4(blk, U0 + D.pb));
+                if (D.t != 0) mma2(u, load_b4(blk, L0 + D.pb), load_b4(blk, U0 + D.pa));
+            }
+            if (D.t == 0) {
+                for (int q = D.q + G; q < D.qe; q += G)
+                    mma2(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
             } else {
-                const int p = Y.colptr[j] + t - 1;
-                B4<T> Lb = load_b4(blk, L0 + p), Ub = load_b4(blk, U0 + p);
-                store_b4(blk, L0 + p, B4<T>{Lb.a - a.a, Lb.b - a.b, Lb.c - a.c, Lb.d - a.d});
-                store_b4(blk, U0 + p, B4<T>{Ub.a - u.a, Ub.b - u.b, Ub.c - u.c, Ub.d - u.d});
+                for (int q = D.q + G; q < D.qe; q += G) {
+                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
+                    mma2(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
+                    mma2(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
+                }
+            }
+            a = group_sum<T, G>(a);
+            u = group_sum<T, G>(u);
+            if (lane == 0) {
+                if (D.t == 0) {
+                    const B4<T> P = load_b4(blk, D.j);
+                    store_b4(blk, D.j, B4<T>{P.a - a.a, P.b - a.b, P.c - a.c, P.d - a.d});
+                } else {
+                    const int p = Y.colptr[D.j] + D.t - 1;
+                    const B4<T> Lb = load_b4(blk, L0 + p), Ub = load_b4(blk, U0 + p);
+                    store_b4(blk, L0 + p, B4<T>{Lb.a - a.a, Lb.b - a.b, Lb.c - a.c, Lb.d - a.d});
+                    store_b4(blk, U0 + p, B4<T>{Ub.a - u.a, Ub.b - u.b, Ub.c - u.c, Ub.d - u.d});
+                }
             }
         }
-    }
-    __syncthreads();
-    const int n0 = Y.lvl_ptr[l], nn = Y.lvl_ptr[l + 1] - n0;
-    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
-        const int j = (n0 + i);
-        B4<T> D = load_b4(blk, j);
-        T det = D.a * D.d - D.b * D.c;
-        if (!(det != (T)0) || !isfinite((double)det)) {
-            atomicOr(bad, 1);
-            det = (T)1;
+        __syncthreads();
+        for (int e = grp; e < ne; e += ng) {
+            if (lane != 0) continue;
+            const Dest D = (e == grp) ? cur : dest(e0, e);
+            const B4<T> P = load_b4(blk, D.j);
+            T det = P.a * P.d - P.b * P.c;
+            const bool sing = !(det != (T)0) || !isfinite((double)det);
+            if (sing) det = (T)1;
+            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
+            if (D.t == 0) {
+                if (sing) atomicOr(bad, 1);
+                store_b4(blk, DI + D.j, Pi);
+            } else {
+                const int p = Y.colptr[D.j] + D.t - 1;
+                const B4<T> lb = mul(load_b4(blk, L0 + p), Pi);
+                store_b4(blk, L0 + p, lb);
+                store_b4(blk, LR0 + Y.lr_pos[p], lb);
+            }
         }
-        store_b4(blk, j, B4<T>{D.d / det, -D.b / det, -D.c / det, D.a / det});
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-        const int j = Y.ld_node[e0 + e], d = Y.ld_d[e0 + e];
-        const int t = d - Y.fd_ptr[j];
-        if (t == 0) continue;
-        const int p = Y.colptr[j] + t - 1;
-        const B4<T> l = mul(load_b4(blk, L0 + p), load_b4(blk, j));
-        store_b4(blk, L0 + p, l);
-        store_b4(blk, U0 + Y.nnzL + Y.lr_pos[p], l);
+        __syncthreads();
+        cur = nxt;
     }
 }
 

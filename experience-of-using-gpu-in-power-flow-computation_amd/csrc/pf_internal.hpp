@@ -83,8 +83,11 @@ struct Symbolic {
     //  slots [0, n): diagonal, [n, n+nnzL): L, [n+nnzL, n+2nnzL): U
     std::vector<int> slot_e;
     int nslot() const { return n + 2 * nnzL; }
-    // slots + the row-ordered copy of L + the dense chain inverse
-    int nblk() const { return n + 3 * nnzL + ch_m * ch_m; }
+    // device factor storage, in blocks: Jacobian slots [0, n + 2 nnzL) (the
+    // diagonal slots keep the updated pivots), the row-ordered copy of L,
+    // the dense chain inverse, then the inverse pivots at di0()
+    int di0() const { return n + 3 * nnzL + ch_m * ch_m; }
+    int nblk() const { return di0() + n; }
     // nodes numbered level by level: level l is the node range
     // [lvl_ptr[l], lvl_ptr[l+1]); the GPU kernels rely on it
     bool level_ordered() const {

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -69,6 +70,8 @@ struct pf_solver {
     std::vector<int> e2slot;
     std::vector<double> Pliq, Qliq, Qload;   // current injections (host copy)
     std::vector<double> hblk, hx;
+    FlowBranch *flow_br = nullptr;   // branch-flow constants (built on first use)
+    double *flow_out = nullptr;
 
     template <typename T>
     pf_status alloc(T **p, size_t n) {
@@ -111,6 +114,7 @@ struct pf_solver {
 
     pf_status build();
     pf_status reset_state();
+    pf_status flows(double *Pdp, double *Qdp, double *Ppd, double *Qpd);
     pf_status set_isq_from_limq(const unsigned char *limQ);
     pf_status reset_ctl();
     pf_status enqueue_head(int gate);
@@ -203,6 +207,7 @@ pf_status pf_solver::build() {
     dy.nnzL = sym.nnzL;
     dy.nslot = sym.nslot();
     dy.nlev = sym.nlev;
+    dy.di0 = sym.di0();
     dy.ch_m = sym.ch_m;
     dy.ch_c0 = sym.ch_c0;
     dy.nlev_s = sym.nlev_s;
@@ -576,6 +581,64 @@ static pf_status stage_state(pf_solver *s, const double *V, const double *theta,
     k_pq<4><<<(s->net.nB + 63) / 64, 256, 0, s->stream>>>(s->dn, s->ds, 0);
     HIPCHK(hipGetLastError());
     return PF_OK;
+}
+
+// Branch flows at the device state (V, theta, Ybus).  The per-branch lookups
+// follow the reference: phif(k,m) (CPU/phifBshf.cpp:31-46 over the triplet
+// matrix of CPU/estrutura/ramo.cpp:4-13, duplicates summed) is the phase of
+// the branches stored as (k,m), else minus that of (m,k), else 0; bshf(k,m)
+// (CPU/phifBshf.cpp:48-63) is the charging of the first branch joining k, m.
+pf_status pf_solver::flows(double *Pdp, double *Qdp, double *Ppd, double *Qpd) {
+    const int nL = net.nL;
+    if (nL == 0) return PF_OK;
+    if (!flow_br) {
+        std::map<std::pair<int, int>, double> phisum;
+        std::map<std::pair<int, int>, int> first;
+        for (int l = 0; l < nL; l++) {
+            phisum[{net.from[l], net.to[l]}] += net.phi[l];
+            auto key = std::minmax(net.from[l], net.to[l]);
+            first.emplace(std::make_pair(key.first, key.second), l);
+        }
+        auto phif = [&](int k, int m) {
+            auto it = phisum.find({k, m});
+            if (it != phisum.end()) return it->second;
+            it = phisum.find({m, k});
+            return it != phisum.end() ? -it->second : 0.0;
+        };
+        auto bshf = [&](int k, int m) {
+            auto key = std::minmax(k, m);
+            auto it = first.find(std::make_pair(key.first, key.second));
+            return it != first.end() ? net.bch[it->second] : 0.0;
+        };
+        std::vector<FlowBranch> h(nL);
+        for (int l = 0; l < nL; l++) {
+            const int f = net.from[l], t = net.to[l];
+            h[l] = FlowBranch{f, t, yp.br_ft[l], yp.br_tf[l], phif(f, t), phif(t, f), bshf(f, t), bshf(t, f)};
+        }
+        PFCHK(upload(&flow_br, h));
+        PFCHK(alloc(&flow_out, 4 * (size_t)nL));
+    }
+    k_flows<<<(nL + 255) / 256, 256, 0, stream>>>(dn, ds, flow_br, nL, flow_out);
+    HIPCHK(hipGetLastError());
+    if (Pdp) PFCHK(get(Pdp, flow_out, nL));
+    if (Qdp) PFCHK(get(Qdp, flow_out + nL, nL));
+    if (Ppd) PFCHK(get(Ppd, flow_out + 2 * (size_t)nL, nL));
+    if (Qpd) PFCHK(get(Qpd, flow_out + 3 * (size_t)nL, nL));
+    HIPCHK(hipStreamSynchronize(stream));
+    return PF_OK;
+}
+
+pf_status pf_solver_flows(pf_solver *s, double *Pdp, double *Qdp, double *Ppd, double *Qpd) {
+    if (!s) return PF_INVALID_ARG;
+    HIPCHK(hipSetDevice(s->device));
+    return s->flows(Pdp, Qdp, Ppd, Qpd);
+}
+
+pf_status pf_stage_flows(pf_solver *s, const double *V, const double *theta, double *Pdp, double *Qdp,
+                         double *Ppd, double *Qpd) {
+    if (!s || !V || !theta) return PF_INVALID_ARG;
+    PFCHK(stage_state(s, V, theta, nullptr, nullptr));
+    return s->flows(Pdp, Qdp, Ppd, Qpd);
 }
 
 pf_status pf_stage_pq(pf_solver *s, const double *V, const double *theta, double *P, double *Q) {

@@ -28,6 +28,8 @@
  *                                        GPU/newtonRaphson.cuh:466-468
  *   pf_stage_limits                      chkLimQ/undo_chkLimQ/geraVetPV/geraVetPQ
  *                                        CPU/limInjReat.cpp:3-131, CPU/newtonRaphson.cpp:64-154
+ *   pf_solver_flows / pf_stage_flows     SpCalcFlux          CPU/esparso/potencia/spCalcFluxPQ.cpp:3-36,
+ *                                        phif/bshf           CPU/phifBshf.cpp:31-63
  *   pf_batch_*                           (no counterpart: the reference solves one case per run)
  *   pf_warmstart_*                       (no counterpart: learned initial state)
  */
@@ -111,6 +113,8 @@ pf_status pf_read_case(const char *path, double v0, double theta0, pf_network **
 void pf_network_free(pf_network *net);
 pf_status pf_network_get_info(const pf_network *net, pf_network_info *info);
 pf_status pf_network_get_bus(const pf_network *net, int field, double *out /* nB */);
+/* bus numbers as written in the file [nB] */
+pf_status pf_network_get_ids(const pf_network *net, int *id);
 /* any pointer may be NULL */
 pf_status pf_network_get_lists(const pf_network *net, int *pv, int *pq, double *qmin,
                                double *qmax, double *vset);
@@ -152,6 +156,13 @@ pf_status pf_solver_lu_info(pf_solver *s, int *n, int *nnzL, int *levels, int *t
 pf_status pf_stage_ybus(pf_solver *s, int *rowptr, int *colind, double *y, int *nnz);
 pf_status pf_stage_pq(pf_solver *s, const double *V, const double *theta, double *P,
                       double *Q);
+/* Branch flows as the reference reports them, per branch [nL]: Pdp/Qdp from
+ * -> to, Ppd/Qpd to -> from (any output may be NULL).  The formula and its
+ * lookups are the reference's, including the sign of the flows and the full
+ * line charging in Q.  pf_solver_flows: at the solver's current state.      */
+pf_status pf_stage_flows(pf_solver *s, const double *V, const double *theta, double *Pdp,
+                         double *Qdp, double *Ppd, double *Qpd);
+pf_status pf_solver_flows(pf_solver *s, double *Pdp, double *Qdp, double *Ppd, double *Qpd);
 /* limQ [nPV] and QliqLim [nB] give the PV->PQ state; g gets nB-1+nPQlim
  * entries in the reference's order; returns max|g|.                         */
 pf_status pf_stage_mismatch(pf_solver *s, const double *V, const double *theta,
@@ -244,7 +255,9 @@ pf_status pf_batch_get_shape(pf_batch *b, int net, int *mode, int *block, int *g
 /* ------------------------------------------------------------ warm start */
 /* MLP: injections + setpoints -> |V|, theta.  BF16 weights/activations,
  * FP32 accumulation on the matrix cores, FP32 master weights and AdamW.
- * Parameter layout: per layer W [K x N] row-major then bias [N].           */
+ * Parameter layout: per layer W [K x N] row-major then bias [N].
+ * n_in, n_out and hidden must be multiples of 8 (16-byte rows of bf16);
+ * pad features and targets with zero columns otherwise.                    */
 typedef struct pf_warmstart pf_warmstart;
 pf_status pf_warmstart_create(int n_in, int n_out, int hidden, int n_hidden_layers,
                               unsigned long long seed, int device, pf_warmstart **out);

@@ -56,6 +56,7 @@ def lib():
         L.orc_limits.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.orc_nr.argtypes = [P, C.POINTER(OrcOpts), P, P, P, P, P, P, C.POINTER(OrcResult)]
         L.orc_nr_batch.argtypes = [P, C.POINTER(OrcOpts), C.c_int, P, P, P, P, P, P, P, C.c_int]
+        L.orc_flows.argtypes = [P, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -125,6 +126,13 @@ class Case:
         a = [np.ascontiguousarray(x, np.float64) for x in (Qload, Qliq, Q, V)]
         f = lib().orc_limits(self.h, mode, *[_ptr(x) for x in a], _ptr(lq), _ptr(ql))
         return lq.astype(bool), ql, f
+
+    def flows(self, V, theta):
+        """Branch flows as the reference reports them: (Pdp, Qdp, Ppd, Qpd) [nL]."""
+        V, theta = (np.ascontiguousarray(a, np.float64) for a in (V, theta))
+        out = [np.zeros(self.nL) for _ in range(4)]
+        lib().orc_flows(self.h, _ptr(V), _ptr(theta), *[_ptr(a) for a in out])
+        return tuple(out)
 
     def solve_batch(self, Pliq, Qliq, Qload, tol=1e-8, max_iter=100, lim=True,
                     reuse_symbolic=False, nthreads=1):

@@ -1047,3 +1047,62 @@ int orc_nr_batch(const orc_case *c, const orc_opts *o, int nsc, const double *Pl
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Branch flows as the reference reports them                                 */
+/* SpCalcFlux CPU/esparso/potencia/spCalcFluxPQ.cpp:17-36, SpP/SpQ :3-15:     */
+/*   a = phif(k,m) + th_k - th_m, G + jB = Ybus(k,m)                          */
+/*   P_km = V_k^2 G - V_k V_m G cos a - V_k V_m B sin a                       */
+/*   Q_km = -V_k^2 (B + bshf(k,m)) + V_k V_m B cos a - V_k V_m G sin a        */
+/* phif CPU/phifBshf.cpp:31-46 over eigen_phi, built from the (de, para, phi) */
+/* triplets with duplicates summed (CPU/estrutura/ramo.cpp:4-13): the stored  */
+/* (k,m) entry, else minus the stored (m,k) entry, else 0.  bshf              */
+/* CPU/phifBshf.cpp:48-63: bsh of the first branch joining k and m.           */
+/* Ybus(k,m) is 0 where the pattern has no entry (Eigen coeff()).             */
+/* ------------------------------------------------------------------------- */
+
+static double flow_phif(const orc_case *c, int k, int m) {
+    double s = 0;
+    int found = 0;
+    for (int l = 0; l < c->nL; l++)
+        if (c->from[l] == k && c->to[l] == m) { s += c->phi[l]; found = 1; }
+    if (found) return s;
+    for (int l = 0; l < c->nL; l++)
+        if (c->from[l] == m && c->to[l] == k) { s += c->phi[l]; found = 1; }
+    return found ? -s : 0.0;
+}
+
+static double flow_bshf(const orc_case *c, int a, int b) {
+    for (int l = 0; l < c->nL; l++) {
+        if (c->from[l] == a) {
+            if (c->to[l] == b) return c->bch[l];
+        } else if (c->to[l] == a && c->from[l] == b) {
+            return c->bch[l];
+        }
+    }
+    return 0.0;
+}
+
+static void flow_km(const orc_case *c, const int *rowptr, const int *colind, const double *y,
+                    const double *V, const double *th, int k, int m, double *P, double *Q) {
+    double G = 0, B = 0;
+    for (int e = rowptr[k]; e < rowptr[k + 1]; e++)
+        if (colind[e] == m) { G = y[2 * e]; B = y[2 * e + 1]; break; }
+    const double a = flow_phif(c, k, m) + th[k] - th[m];
+    const double vkk = V[k] * V[k], vkm = V[k] * V[m];
+    *P = vkk * G - vkm * G * cos(a) - vkm * B * sin(a);
+    *Q = -vkk * (B + flow_bshf(c, k, m)) + vkm * B * cos(a) - vkm * G * sin(a);
+}
+
+void orc_flows(const orc_case *c, const double *V, const double *theta, double *Pdp, double *Qdp,
+               double *Ppd, double *Qpd) {
+    int nnz = orc_ybus_nnz(c);
+    int *rowptr = malloc(sizeof(int) * (c->nB + 1)), *colind = malloc(sizeof(int) * (nnz > 0 ? nnz : 1));
+    double *y = malloc(sizeof(double) * 2 * (nnz > 0 ? nnz : 1));
+    orc_ybus(c, rowptr, colind, y);
+    for (int l = 0; l < c->nL; l++) {
+        flow_km(c, rowptr, colind, y, V, theta, c->from[l], c->to[l], &Pdp[l], &Qdp[l]);
+        flow_km(c, rowptr, colind, y, V, theta, c->to[l], c->from[l], &Ppd[l], &Qpd[l]);
+    }
+    free(rowptr); free(colind); free(y);
+}

@@ -97,6 +97,11 @@ int  orc_nr(const orc_case *c, const orc_opts *o, const double *Pliq,
 
 /* scenario batch on the CPU: nsc scenarios of the same network, each with its */
 /* own Pliq/Qliq/Qload rows; scenario-parallel over nthreads OpenMP threads.  */
+/* --- branch flows as reported (CPU/esparso/potencia/spCalcFluxPQ.cpp:3-36) -- */
+/* Pdp/Qdp: from -> to, Ppd/Qpd: to -> from, per branch [nL]                  */
+void orc_flows(const orc_case *c, const double *V, const double *theta, double *Pdp, double *Qdp,
+               double *Ppd, double *Qpd);
+
 int  orc_nr_batch(const orc_case *c, const orc_opts *o, int nsc, const double *Pliq,
                   const double *Qliq, const double *Qload, double *V, double *theta,
                   int *iters, int *converged, int nthreads);

@@ -77,6 +77,35 @@ def test_pq(pf, oracle, cases, name):
         assert np.abs(Q - Qo).max() <= 1e-11 * scale
 
 
+@pytest.mark.parametrize("name", ["c14", "ieee57", "ieee118", "c1888"])
+def test_flows(pf, oracle, cases, name):
+    """Reported branch flows (the reference's formula and lookups) on the GPU
+    against the oracle: at random states through the stage entry point, and at
+    the solver's own answer.  Tolerance 1e-11 x (1 + max|flow|): the GPU's sin
+    and cos are not the host's."""
+    p = _path(cases, name)
+    _say("flows", name)
+    net = pf.Network(p)
+    oc = oracle.Case(p)
+    s = _solver(pf, net)
+    for seed in range(2):
+        V, th = _rand_state(net, seed)
+        got = s.flows(V, th)
+        ref = oc.flows(V, th)
+        scale = 1 + max(np.abs(r).max() for r in ref)
+        for g, e in zip(got, ref):
+            assert np.abs(g - e).max() <= 1e-11 * scale
+    s.reset()   # the stage calls left a random state on the device
+    r = s.solve()
+    assert r["converged"]
+    V, th, _ = s.state()
+    got = s.flows()
+    ref = oc.flows(V, th)
+    scale = 1 + max(np.abs(x).max() for x in ref)
+    for g, e in zip(got, ref):
+        assert np.abs(g - e).max() <= 1e-11 * scale
+
+
 @pytest.mark.parametrize("name", ["c14", "c118", "ieee118"])
 def test_mismatch(pf, oracle, cases, name):
     p = _path(cases, name)

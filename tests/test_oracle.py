@@ -336,6 +336,69 @@ def test_limit_rules_known_answers(oracle):
 
 # ------------------------------------------------------ warm-start MLP oracle
 
+def _flows_closed_form(c, V, th, phase=None):
+    """Flows of every branch from its own parameters (numpy complex), in the
+    reference's convention: with Y_km the branch's Ybus term, psi_km the phase
+    added by phif (+phi from the from side, -phi from the to side), and the
+    full line charging b:  S = conj(Y_km) (V_k^2 - V_k V_m e^{j(th_k - th_m + psi_km)}),
+    P = Re S, Q = Im S - V_k^2 b."""
+    out = [np.zeros(c.nL) for _ in range(4)]
+    for l in range(c.nL):
+        f, t = c.frm[l], c.to[l]
+        y = 1.0 / complex(c.r[l], c.x[l])
+        tap = complex(c.tap_re[l], c.tap_im[l])
+        for (k, m, Ykm, psi, P, Q) in ((f, t, -y / np.conj(tap), c.phi[l], out[0], out[1]),
+                                      (t, f, -y / tap, -c.phi[l], out[2], out[3])):
+            S = np.conj(Ykm) * (V[k] ** 2 - V[k] * V[m] * np.exp(1j * (th[k] - th[m] + psi)))
+            P[l] = S.real
+            Q[l] = S.imag - V[k] ** 2 * c.bch[l]
+    return out
+
+
+@pytest.mark.parametrize("name", ["ieee14", "ieee30", "c118"])
+def test_flows_closed_form(oracle, name):
+    """Reported branch flows against the closed form from the branch data (the
+    reference's sign and its full line charging in Q), at the NR solution;
+    branches sharing a bus pair are skipped (their Ybus term is the sum)."""
+    path = os.path.join(IEEE, name + ".cdf") if name.startswith("ieee") else os.path.join(GOLDEN, name + ".cdf")
+    c = oracle.Case(path)
+    r = c.solve(tol=1e-10, lim=False)
+    assert r["converged"]
+    got = c.flows(r["V"], r["theta"])
+    ref = _flows_closed_form(c, r["V"], r["theta"])
+    pairs = [tuple(sorted(p)) for p in zip(c.frm, c.to)]
+    alone = np.array([pairs.count(p) == 1 for p in pairs])
+    assert alone.sum() >= c.nL - 8
+    for g, e in zip(got, ref):
+        assert np.abs(g[alone] - e[alone]).max() <= 1e-12
+    # the reported flow is minus the usual k -> m flow: Pdp + Ppd = -(series losses)
+    E = r["V"] * np.exp(1j * r["theta"])
+    for l in np.where(alone & (c.tap_re == 1) & (c.tap_im == 0))[0]:
+        f, t = c.frm[l], c.to[l]
+        loss = (abs(E[f] - E[t]) ** 2 / complex(c.r[l], c.x[l]).conjugate()).real
+        assert abs(got[0][l] + got[2][l] + loss) <= 1e-12
+
+
+def test_flows_phase_shifter(oracle, tmp_path):
+    """A 5-degree phase shift on branch 4-7 of ieee14: phif adds +phi from the
+    from side and -phi from the to side, on top of the phase already in Ybus."""
+    lines = open(os.path.join(IEEE, "ieee14.cdf"), "rb").read().decode().split("\n")
+    k = next(i for i, l in enumerate(lines) if l.startswith("   4    7"))
+    lines[k] = lines[k][:83] + f"{5.0:<7.1f}" + lines[k][90:]
+    p = tmp_path / "ps14.cdf"
+    p.write_bytes("\n".join(lines).encode())
+    c = oracle.Case(str(p))
+    l = int(np.where((c.frm == 3) & (c.to == 6))[0][0])
+    assert abs(c.phi[l] - np.deg2rad(5.0)) < 1e-15
+    rng = np.random.default_rng(5)
+    V = 1 + 0.05 * rng.standard_normal(c.nB)
+    th = 0.1 * rng.standard_normal(c.nB)
+    got = c.flows(V, th)
+    ref = _flows_closed_form(c, V, th)
+    for g, e in zip(got, ref):
+        assert np.abs(g - e).max() <= 1e-12
+
+
 def test_mlp_oracle_gradient_fd():
     """The MLP oracle's backward pass against central differences (FP64)."""
     import mlp_oracle as M

@@ -166,6 +166,39 @@ def warm_start_leg(pf, synth, net, opts, n_train, n_eval, epochs, seed_train, se
             "max_abs_dV_warm_vs_flat_pu": float(np.abs(Vw[both] - Vf[both]).max()) if both.any() else None}
 
 
+def single_leg(pf, synth, tmp, variants, refine):
+    """The single 15,104-bus solve (flat start, Q limits, tol 1e-8), NR loop
+    time median of 3 after a warm-up solve, for each requested variant:
+    plain_fp64_eager (one launch per elimination-tree level, host sync per
+    iteration), fp64_eager / fp64_graph (subtree groups + dense separator
+    block, eager or one captured graph per iteration), mixed_eager /
+    mixed_graph (the same with the FP32 factor and FP64 refinement)."""
+    kinds = {"plain_fp64_eager": dict(lu_blocked=0), "fp64_eager": dict(), "fp64_graph": dict(use_graph=1),
+             "mixed_eager": dict(precision=pf.MIXED, refine_steps=refine),
+             "mixed_graph": dict(use_graph=1, precision=pf.MIXED, refine_steps=refine)}
+    p15 = os.path.join(tmp, "c15104.cdfx")
+    synth.write_cdf(synth.case15104(), p15)
+    n15 = pf.Network(p15)
+    res = {}
+    s15 = None
+    for tag in variants:
+        s15 = pf.Solver(n15, pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, **kinds[tag]))
+        s15.solve()
+        times = []
+        for _ in range(3):
+            s15.reset()
+            r = s15.solve()
+            times.append(r["solve_ms"])
+        res[tag] = {"ms": round(float(np.median(times)), 3), "iterations": r["iterations"],
+                    "converged": r["converged"]}
+        log(f"single 15,104-bus {tag}: {res[tag]['ms']} ms")
+    if "plain_fp64_eager" in res and "mixed_graph" in res:
+        res["speedup_mixed_graph_vs_plain"] = round(res["plain_fp64_eager"]["ms"] / res["mixed_graph"]["ms"], 2)
+    if s15 is not None:
+        res["lu"] = s15.lu_info()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +209,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single 15,104-bus solve")
+    ap.add_argument("--single-only", action="store_true", help="only the single 15,104-bus solve")
+    ap.add_argument("--single-variants", default="plain_fp64_eager,fp64_graph,mixed_graph")
     ap.add_argument("--refine", type=int, default=2)
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start legs")
     ap.add_argument("--warm-train", type=int, default=50000)
@@ -205,6 +240,9 @@ def main():
         sys.exit(2)
     pf, synth = load_pkg()
     tmp = tempfile.mkdtemp(prefix="pfbench_")
+    if args.single_only:
+        print(json.dumps({"single_15104": single_leg(pf, synth, tmp, args.single_variants.split(","), args.refine)}))
+        return
     p118 = os.path.join(tmp, "c118.cdf")
     p1888 = os.path.join(tmp, "c1888.cdf")
     synth.write_cdf(synth.case118(), p118)
@@ -296,27 +334,7 @@ def main():
 
     if not args.no_extras and not args.no_single:
         log("single 15,104-bus solve")
-        # single 15,104-bus solve
-        p15 = os.path.join(tmp, "c15104.cdfx")
-        synth.write_cdf(synth.case15104(), p15)
-        n15 = pf.Network(p15)
-        res = {}
-        for tag, kw in (("plain_fp64_eager", dict(lu_blocked=0)),
-                        ("fp64_graph", dict(use_graph=1)),
-                        ("mixed_graph", dict(use_graph=1, precision=pf.MIXED, refine_steps=args.refine))):
-            s15 = pf.Solver(n15, pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, **kw))
-            s15.solve()
-            times = []
-            for _ in range(3):
-                s15.reset()
-                r = s15.solve()
-                times.append(r["solve_ms"])
-            res[tag] = {"ms": round(float(np.median(times)), 3), "iterations": r["iterations"],
-                        "converged": r["converged"]}
-        res["speedup_mixed_graph_vs_plain"] = round(res["plain_fp64_eager"]["ms"] / res["mixed_graph"]["ms"], 2)
-        lu = s15.lu_info()
-        res["lu"] = lu
-        extras["single_15104"] = res
+        extras["single_15104"] = single_leg(pf, synth, tmp, args.single_variants.split(","), args.refine)
 
     if not args.no_extras and not args.no_warm:
         # learned warm start: the 944-bus 8-copy network (50k training, 4096

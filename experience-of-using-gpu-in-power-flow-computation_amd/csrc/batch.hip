@@ -121,20 +121,18 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
         int p = -1;
         if (t < nd) {
             const int d = d0 + t;
+            B4<T> sa{0, 0, 0, 0}, su{0, 0, 0, 0};
             if (t == 0) {
+                pairs_acc<T, false, 2>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
                 a = load_b4(blk, j);
-                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++)
-                    mms(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
             } else {
+                pairs_acc<T, true, 2>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
                 p = Y.colptr[j] + t - 1;
                 a = load_b4(blk, L0 + p);
                 u = load_b4(blk, U0 + p);
-                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++) {
-                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
-                    mms(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
-                    mms(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
-                }
             }
+            a = B4<T>{a.a - sa.a, a.b - sa.b, a.c - sa.c, a.d - sa.d};
+            u = B4<T>{u.a - su.a, u.b - su.b, u.c - su.c, u.d - su.d};
         }
         if (base == 0) {
             B4<T> di{0, 0, 0, 0};
@@ -516,12 +514,12 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
                 } else {   // a run of narrow levels
                     int l1 = l + 1;
                     while (l1 < Y.nlev_s && Y.lvl_ptr[l1 + 1] - Y.lvl_ptr[l1] < kWideLevel) l1++;
-                    coop_factor_range<T, GL>(Y, w.blk, l, l1, &s_sing);
+                    coop_factor_range<T, kCoopLanes, 2>(Y, w.blk, l, l1, &s_sing);
                     l = l1;
                 }
             }
             if (Y.ch_m > 0) {
-                chain_factor_mem<T, GL, kBatchChain>(Y, w.blk, &s_sing);
+                chain_factor_mem<T, GL, 2, kBatchChain>(Y, w.blk, &s_sing);
                 __syncthreads();
             }
         }
@@ -582,8 +580,8 @@ __host__ __device__ inline long long partial_lds_bytes(int n, int nlev) {
 // MODE 1: working set in a global slice, except the step vectors and the
 //         level / row-structure metadata, staged in LDS (latency of the
 //         level-by-level walk); MODE 0: everything in the global slice.
-template <typename T, int MODE>
-__global__ void __launch_bounds__(1024) k_batch(BatchTopo tp0, BatchJob J) {
+template <typename T, int MODE, int MAXT>   // MAXT: largest workgroup (register budget)
+__global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_item, s_it, s_conv;
     __shared__ double s_maxg;
@@ -644,6 +642,24 @@ __global__ void __launch_bounds__(1024) k_batch(BatchTopo tp0, BatchJob J) {
 __global__ void k_zero_int(int *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// The instantiation for a precision, working-set placement and workgroup
+// size: compiled for at most 256, 512 or 1024 threads, so that smaller
+// workgroups get the larger register budget.
+using BatchKernel = void (*)(BatchTopo, BatchJob);
+template <typename T, int MODE>
+BatchKernel batch_kernel_t(int block) {
+    if (block <= 256) return k_batch<T, MODE, 256>;
+    if (block <= 512) return k_batch<T, MODE, 512>;
+    return k_batch<T, MODE, 1024>;
+}
+inline BatchKernel batch_kernel(bool mixed, int mode, int block) {
+    if (mixed)
+        return mode == 2 ? batch_kernel_t<float, 2>(block)
+                         : mode == 1 ? batch_kernel_t<float, 1>(block) : batch_kernel_t<float, 0>(block);
+    return mode == 2 ? batch_kernel_t<double, 2>(block)
+                     : mode == 1 ? batch_kernel_t<double, 1>(block) : batch_kernel_t<double, 0>(block);
 }
 
 }  // namespace pf
@@ -826,10 +842,11 @@ pf_status pf_batch::add_net(const pf_network &src) {
     } else {
         long long pb = mixed ? partial_lds_bytes<float>(n, sym.nlev) : partial_lds_bytes<double>(n, sym.nlev);
         hipError_t e = hipSuccess;
-        if (pb <= max_lds) {
-            if (mixed) e = hipFuncSetAttribute((const void *)k_batch<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pb);
-            else e = hipFuncSetAttribute((const void *)k_batch<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pb);
-        }
+        if (pb <= max_lds)
+            for (int blk : {256, 512, 1024})
+                if (e == hipSuccess)
+                    e = hipFuncSetAttribute((const void *)batch_kernel(mixed, 1, blk),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pb);
         if (pb <= max_lds && e == hipSuccess) {
             b->partial = pb;
             b->mode = 1;
@@ -850,17 +867,8 @@ pf_status pf_batch::occupancy(const BatchNet &nb, int mode, int block, int *per_
     const bool mixed = opt.precision != PF_FP64;
     const size_t shm = mode == 2 ? (size_t)nb.ws : mode == 1 ? (size_t)nb.partial : 0;
     int pc = 0;
-    hipError_t e;
-    if (mixed) {
-        if (mode == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<float, 2>, block, shm);
-        else if (mode == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<float, 1>, block, shm);
-        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<float, 0>, block, shm);
-    } else {
-        if (mode == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<double, 2>, block, shm);
-        else if (mode == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<double, 1>, block, shm);
-        else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, k_batch<double, 0>, block, shm);
-    }
-    BHIP(e);
+    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, (const void *)batch_kernel(mixed, mode, block), block,
+                                                      shm));
     *per_cu = pc;
     return PF_OK;
 }
@@ -915,16 +923,10 @@ pf_status pf_batch::launch_net(size_t i, hipStream_t s) {
         J.tol = opt.tolerancia;
         J.diverge = 1e6;
         const size_t shm = (size_t)b.shm;
-        if (mixed) {
-            if (b.mode == 2) k_batch<float, 2><<<b.grid, b.block, shm, s>>>(b.tp, J);
-            else if (b.mode == 1) k_batch<float, 1><<<b.grid, b.block, shm, s>>>(b.tp, J);
-            else k_batch<float, 0><<<b.grid, b.block, 0, s>>>(b.tp, J);
-        } else {
-            if (b.mode == 2) k_batch<double, 2><<<b.grid, b.block, shm, s>>>(b.tp, J);
-            else if (b.mode == 1) k_batch<double, 1><<<b.grid, b.block, shm, s>>>(b.tp, J);
-            else k_batch<double, 0><<<b.grid, b.block, 0, s>>>(b.tp, J);
-        }
-        BHIP(hipGetLastError());
+        BatchTopo tp = b.tp;
+        void *args[] = {&tp, &J};
+        BHIP(hipLaunchKernel((const void *)batch_kernel(mixed, b.mode, b.block), dim3(b.grid), dim3(b.block), args,
+                             b.mode == 0 ? 0 : shm, s));
     }
     return PF_OK;
 }

@@ -282,20 +282,18 @@ __device__ __forceinline__ void factor_node(const DevSym &Y, T *blk, int j, int 
         int p = -1;
         if (t < nd) {
             const int d = d0 + t;
+            B4<T> sa{0, 0, 0, 0}, su{0, 0, 0, 0};
             if (t == 0) {
+                pairs_acc<T, false, 4>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
                 a = load_b4(blk, j);
-                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++)
-                    mms(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
             } else {
+                pairs_acc<T, true, 4>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
                 p = Y.colptr[j] + t - 1;
                 a = load_b4(blk, L0 + p);
                 u = load_b4(blk, U0 + p);
-                for (int q = Y.fc_ptr[d]; q < Y.fc_ptr[d + 1]; q++) {
-                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
-                    mms(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
-                    mms(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
-                }
             }
+            a = B4<T>{a.a - sa.a, a.b - sa.b, a.c - sa.c, a.d - sa.d};
+            u = B4<T>{u.a - su.a, u.b - su.b, u.c - su.c, u.d - su.d};
         }
         if (base == 0) {
             B4<T> di{0, 0, 0, 0};
@@ -333,8 +331,22 @@ __global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    coop_factor_range<T, 16>(Y, blk, lv0, lv1, &ctl->singular);
-    if (Y.ch_m > 0) chain_factor<T, 16, kChainRegs, kChainMax>(Y, blk, &ctl->singular);
+    coop_factor_range<T, kCoopLanes, 4>(Y, blk, lv0, lv1, &ctl->singular);
+}
+
+// Schur complement of the trailing dense chain: every destination is
+// independent, so the pass is spread over many workgroups (16 lanes each).
+template <typename T>
+__global__ void __launch_bounds__(256) k_chain_schur(DevSym Y, T *blk, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    chain_schur<T, 16, 4>(Y, blk, (int)blockIdx.x * 16, (int)gridDim.x * 16);
+}
+
+// Gauss-Jordan inverse of the chain's Schur complement, in one workgroup.
+template <typename T>
+__global__ void __launch_bounds__(ChainTiles<T>::threads) k_chain_invert(DevSym Y, T *blk, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    chain_invert<T, ChainTiles<T>::tr, ChainTiles<T>::tc, kChainMax>(Y, blk, &ctl->singular);
 }
 
 // ----------------------------------------------------- triangular solves
@@ -403,41 +415,46 @@ __global__ void __launch_bounds__(256) k_bwd_level(DevSym Y, const T *blk, doubl
 // Levels below the tail in one launch: workgroup g walks the levels of its
 // own group of subtrees (every update of a node comes from its descendants,
 // so the groups never wait on each other).
+// 1024 threads: 16 wavefronts (a node each) in the factorisation, 64 lane
+// groups of 16 (a node each) in the sweeps.
 template <typename T>
-__global__ void __launch_bounds__(256) k_factor_parts(DevSym Y, T *blk, Ctl *ctl, int gate) {
+__global__ void __launch_bounds__(1024) k_factor_parts(DevSym Y, T *blk, Ctl *ctl, int gate) {
     if (gate && !ctl->step) return;
     const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63, nl = Y.part_levels;
+    const int nw = (int)blockDim.x / 64;
     const int *pp = Y.part_ptr + (size_t)blockIdx.x * nl;
     for (int l = 0; l < nl; l++) {
         const int s = pp[l], e = pp[l + 1];
         if (s == e) continue;
-        for (int i = s + w; i < e; i += 4) factor_node(Y, blk, Y.part_nodes[i], lane, &ctl->singular);
+        for (int i = s + w; i < e; i += nw) factor_node(Y, blk, Y.part_nodes[i], lane, &ctl->singular);
         __syncthreads();
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate) {
+__global__ void __launch_bounds__(1024) k_fwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate) {
     if (gate && !ctl->step) return;
     const int g = (int)threadIdx.x / 16, lane = (int)threadIdx.x & 15, nl = Y.part_levels;
+    const int ng = (int)blockDim.x / 16;
     const int *pp = Y.part_ptr + (size_t)blockIdx.x * nl;
     for (int l = 0; l < nl; l++) {
         const int s = pp[l], e = pp[l + 1];
         if (s == e) continue;
-        for (int i = s + g; i < e; i += 16) fwd_node(Y, blk, x, Y.part_nodes[i], lane);
+        for (int i = s + g; i < e; i += ng) fwd_node(Y, blk, x, Y.part_nodes[i], lane);
         __syncthreads();
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate) {
+__global__ void __launch_bounds__(1024) k_bwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate) {
     if (gate && !ctl->step) return;
     const int g = (int)threadIdx.x / 16, lane = (int)threadIdx.x & 15, nl = Y.part_levels;
+    const int ng = (int)blockDim.x / 16;
     const int *pp = Y.part_ptr + (size_t)blockIdx.x * nl;
     for (int l = nl - 1; l >= 0; l--) {
         const int s = pp[l], e = pp[l + 1];
         if (s == e) continue;
-        for (int i = s + g; i < e; i += 16) bwd_node(Y, blk, x, Y.part_nodes[i], lane);
+        for (int i = s + g; i < e; i += ng) bwd_node(Y, blk, x, Y.part_nodes[i], lane);
         __syncthreads();
     }
 }
@@ -787,6 +804,10 @@ template __global__ void k_fwd_parts<double>(DevSym, const double *, double *, C
 template __global__ void k_fwd_parts<float>(DevSym, const float *, double *, Ctl *, int);
 template __global__ void k_bwd_parts<double>(DevSym, const double *, double *, Ctl *, int);
 template __global__ void k_bwd_parts<float>(DevSym, const float *, double *, Ctl *, int);
+template __global__ void k_chain_schur<double>(DevSym, double *, Ctl *, int);
+template __global__ void k_chain_schur<float>(DevSym, float *, Ctl *, int);
+template __global__ void k_chain_invert<double>(DevSym, double *, Ctl *, int);
+template __global__ void k_chain_invert<float>(DevSym, float *, Ctl *, int);
 template __global__ void k_factor_tail<double>(DevSym, double *, Ctl *, int, int, int);
 template __global__ void k_factor_tail<float>(DevSym, float *, Ctl *, int, int, int);
 template __global__ void k_fwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);

@@ -58,10 +58,19 @@ struct DevState {
 };
 
 // trailing dense chain of the single-network solver: at most kChainMax
-// nodes (pf_internal.hpp), S held in kChainRegs register blocks per thread
-// of a 1024-thread workgroup
-constexpr int kChainRegs = 10;
-static_assert(kChainMax * kChainMax <= 1024 * kChainRegs, "dense chain does not fit the registers");
+// nodes (pf_internal.hpp), S held in TR x TC register tiles of blocks by one
+// workgroup of 1024 threads
+template <typename T> struct ChainTiles;
+template <> struct ChainTiles<float> { static constexpr int threads = 1024, tr = 4, tc = 3; };
+template <> struct ChainTiles<double> { static constexpr int threads = 1024, tr = 4, tc = 3; };
+template <typename T>
+constexpr bool chain_tiles_fit() {
+    return ((kChainMax + ChainTiles<T>::tr - 1) / ChainTiles<T>::tr) *
+               ((kChainMax + ChainTiles<T>::tc - 1) / ChainTiles<T>::tc) <= ChainTiles<T>::threads;
+}
+static_assert(chain_tiles_fit<float>() && chain_tiles_fit<double>(), "dense chain does not fit the registers");
+// lanes per destination block in the cooperative narrow-level factorisation
+constexpr int kCoopLanes = 4;
 
 // Per-branch constants of the reported flows: the Ybus slots of (from, to)
 // and (to, from), and the reference's phif / bshf lookups in both directions.

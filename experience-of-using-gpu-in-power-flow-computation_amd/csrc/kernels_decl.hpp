@@ -15,6 +15,8 @@ template <typename T> __global__ void k_jfill(DevNet N, DevState S, DevSym Y, T 
 template <typename T> __global__ void k_factor_level(DevSym Y, T *blk, Ctl *ctl, int l0, int count, int gate);
 template <typename T> __global__ void k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate);
 template <typename T> __global__ void k_factor_parts(DevSym Y, T *blk, Ctl *ctl, int gate);
+template <typename T> __global__ void k_chain_schur(DevSym Y, T *blk, Ctl *ctl, int gate);
+template <typename T> __global__ void k_chain_invert(DevSym Y, T *blk, Ctl *ctl, int gate);
 __global__ void k_flows(DevNet N, DevState S, const FlowBranch *br, int nL, double *out);
 template <typename T> __global__ void k_fwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate);
 template <typename T> __global__ void k_bwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate);

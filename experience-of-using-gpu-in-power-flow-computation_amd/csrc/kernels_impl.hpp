@@ -65,6 +65,45 @@ __device__ __forceinline__ B4<T> group_sum(B4<T> v) {
     return v;
 }
 
+// Sums the update pairs q, q + step, ... < qe of one destination block:
+// a += L(p_ik) U(p_jk) and, for a column destination, u += L(p_jk) U(p_ik).
+// UNR pairs at a time: their indices are loaded first, then all their
+// blocks, so a lane waits for one round of dependent loads per UNR pairs
+// instead of one per pair; the sum is taken in pair order either way.
+template <typename T, bool WITH_U, int UNR>
+__device__ __forceinline__ void pairs_acc(const DevSym &Y, const T *blk, int q, int qe, int step, B4<T> &a,
+                                          B4<T> &u) {
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL;
+    for (; q + (UNR - 1) * step < qe; q += UNR * step) {
+        int pa[UNR], pb[UNR];
+#pragma unroll
+        for (int r = 0; r < UNR; r++) {
+            pa[r] = Y.fc_a[q + r * step];
+            pb[r] = Y.fc_b[q + r * step];
+        }
+        B4<T> la[UNR], ub[UNR], lb[UNR], ua[UNR];
+#pragma unroll
+        for (int r = 0; r < UNR; r++) {
+            la[r] = load_b4(blk, L0 + pa[r]);
+            ub[r] = load_b4(blk, U0 + pb[r]);
+            if (WITH_U) {
+                lb[r] = load_b4(blk, L0 + pb[r]);
+                ua[r] = load_b4(blk, U0 + pa[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < UNR; r++) {
+            mma2(a, la[r], ub[r]);
+            if (WITH_U) mma2(u, lb[r], ua[r]);
+        }
+    }
+    for (; q < qe; q += step) {
+        const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
+        mma2(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
+        if (WITH_U) mma2(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
+    }
+}
+
 // Narrow elimination-tree levels [l0, l1) factored by a whole workgroup:
 // groups of G lanes take a level's destination blocks (every node's pivot and
 // column entries), the lanes of a group split that destination's update pairs
@@ -76,7 +115,7 @@ __device__ __forceinline__ B4<T> group_sum(B4<T> v) {
 // level) keeps every lane busy.  The indices of a group's first destination
 // and of each lane's first pair on the next level do not depend on values and
 // are loaded one level ahead.
-template <typename T, int G>
+template <typename T, int G, int UNR>
 __device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *bad) {
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int grp = (int)threadIdx.x / G, lane = (int)threadIdx.x % G, ng = (int)blockDim.x / G;
@@ -110,16 +149,8 @@ __device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *
                 mma2(a, load_b4(blk, L0 + D.pa), load_b4(blk, U0 + D.pb));
                 if (D.t != 0) mma2(u, load_b4(blk, L0 + D.pb), load_b4(blk, U0 + D.pa));
             }
-            if (D.t == 0) {
-                for (int q = D.q + G; q < D.qe; q += G)
-                    mma2(a, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
-            } else {
-                for (int q = D.q + G; q < D.qe; q += G) {
-                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
-                    mma2(a, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
-                    mma2(u, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
-                }
-            }
+            if (D.t == 0) pairs_acc<T, false, UNR>(Y, blk, D.q + G, D.qe, G, a, u);
+            else pairs_acc<T, true, UNR>(Y, blk, D.q + G, D.qe, G, a, u);
             a = group_sum<T, G>(a);
             u = group_sum<T, G>(u);
             if (lane == 0) {
@@ -171,30 +202,23 @@ __device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *
 
 // S = chain block minus the updates of the nodes before the chain, written
 // row-major (ch_m x ch_m blocks) at block n + 3 nnzL.  Groups of G lanes
-// per destination block; every destination is independent.  The caller
-// synchronises before (every node before the chain factored).
-template <typename T, int G>
-__device__ void chain_schur(const DevSym &Y, T *blk) {
+// per destination block (group group0 + threadIdx.x / G of ngroups, so that
+// one or many workgroups can share the pass); every destination is
+// independent.  The caller orders it after every node before the chain.
+template <typename T, int G, int UNR>
+__device__ void chain_schur(const DevSym &Y, T *blk, int group0, int ngroups) {
     const int m = Y.ch_m, c0 = Y.ch_c0;
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, SB = Y.n + 3 * Y.nnzL;
-    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    const int tid = (int)threadIdx.x;
     {
         const int e0 = Y.ld_ptr[Y.nlev_s], ne = Y.ld_ptr[Y.nlev] - e0, dbase = Y.fd_ptr[c0];
-        const int grp = tid / G, lane = tid % G, ng = bd / G;
+        const int grp = group0 + tid / G, lane = tid % G, ng = ngroups;
         for (int e = grp; e < ne; e += ng) {
             const int j = Y.ld_node[e0 + e], d = Y.ld_d[e0 + e];
             const int t = d - Y.fd_ptr[j], a = j - c0, qend = Y.ch_split[d - dbase];
             B4<T> acc{0, 0, 0, 0}, uacc{0, 0, 0, 0};
-            if (t == 0) {
-                for (int q = Y.fc_ptr[d] + lane; q < qend; q += G)
-                    mma2(acc, load_b4(blk, L0 + Y.fc_a[q]), load_b4(blk, U0 + Y.fc_b[q]));
-            } else {
-                for (int q = Y.fc_ptr[d] + lane; q < qend; q += G) {
-                    const int pik = Y.fc_a[q], pjk = Y.fc_b[q];
-                    mma2(acc, load_b4(blk, L0 + pik), load_b4(blk, U0 + pjk));
-                    mma2(uacc, load_b4(blk, L0 + pjk), load_b4(blk, U0 + pik));
-                }
-            }
+            if (t == 0) pairs_acc<T, false, UNR>(Y, blk, Y.fc_ptr[d] + lane, qend, G, acc, uacc);
+            else pairs_acc<T, true, UNR>(Y, blk, Y.fc_ptr[d] + lane, qend, G, acc, uacc);
             acc = group_sum<T, G>(acc);
             uacc = group_sum<T, G>(uacc);
             if (lane == 0) {
@@ -240,53 +264,92 @@ __device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B
     }
 }
 
-// Forms S and overwrites it with S^-1.  S lives in registers during the
-// elimination: thread t owns blocks t + r*blockDim, r < R (requires
-// ch_m^2 <= R * blockDim); row k and column k go through LDS (double
-// buffered: one barrier per step).  The caller synchronises before and after.
-template <typename T, int G, int R, int MAXM>
-__device__ void chain_factor(const DevSym &Y, T *blk, int *bad) {
+// Overwrites S (formed by chain_schur) with S^-1.  S lives in registers
+// during the elimination, as TR x TC tiles of blocks: thread t owns rows
+// (t / ntc) * TR + a and columns (t % ntc) * TC + c, with ntc = ceil(m / TC)
+// (requires ceil(m / TR) * ceil(m / TC) <= blockDim).  Row k and column k go
+// through LDS (double buffered: one barrier per step); a thread reads TR + TC
+// blocks of them per step, not two per block it owns.  The caller orders it
+// after chain_schur and synchronises after.
+template <typename T, int TR, int TC, int MAXM>
+__device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
     __shared__ B4<T> s_row[2][MAXM], s_col[2][MAXM];
-    const int m = Y.ch_m, mm = m * m, SB = Y.n + 3 * Y.nnzL;
-    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
-    chain_schur<T, G>(Y, blk);
-    __syncthreads();
-    B4<T> reg[R];
-    int ri[R], rj[R];
+    const int m = Y.ch_m, SB = Y.n + 3 * Y.nnzL;
+    const int tid = (int)threadIdx.x;
+    const int ntc = (m + TC - 1) / TC, ntr = (m + TR - 1) / TR;
+    const bool own = tid < ntr * ntc;
+    const int i0 = own ? (tid / ntc) * TR : m, j0 = own ? (tid % ntc) * TC : m;
+    B4<T> reg[TR][TC];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int q = tid + r * bd;
-        ri[r] = q < mm ? q / m : -1;
-        rj[r] = q < mm ? q - ri[r] * m : -1;
-        reg[r] = q < mm ? load_b4(blk, SB + q) : B4<T>{0, 0, 0, 0};
-    }
+    for (int a = 0; a < TR; a++)
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+            reg[a][c] = (i0 + a < m && j0 + c < m) ? load_b4(blk, SB + (i0 + a) * m + j0 + c) : B4<T>{0, 0, 0, 0};
     for (int k = 0; k < m; k++) {
         const int b = k & 1;
+        if (k >= i0 && k < i0 + TR) {   // this thread holds part of row k
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (ri[r] == k) s_row[b][rj[r]] = reg[r];
-            if (rj[r] == k) s_col[b][ri[r]] = reg[r];
+            for (int a = 0; a < TR; a++)
+                if (i0 + a == k) {
+#pragma unroll
+                    for (int c = 0; c < TC; c++)
+                        if (j0 + c < m) s_row[b][j0 + c] = reg[a][c];
+                }
+        }
+        if (k >= j0 && k < j0 + TC) {   // ... of column k
+#pragma unroll
+            for (int c = 0; c < TC; c++)
+                if (j0 + c == k) {
+#pragma unroll
+                    for (int a = 0; a < TR; a++)
+                        if (i0 + a < m) s_col[b][i0 + a] = reg[a][c];
+                }
         }
         __syncthreads();
         const B4<T> P = pivot_inverse(s_row[b][k], bad);
+        // column by column: one P S_kj live at a time, S_ik re-read from LDS
 #pragma unroll
-        for (int r = 0; r < R; r++)
-            if (ri[r] >= 0) gj_update(reg[r], ri[r], rj[r], k, P, s_row[b][rj[r]], s_col[b][ri[r]]);
+        for (int c = 0; c < TC; c++) {
+            const int j = j0 + c;
+            if (j >= m) continue;
+            const B4<T> pr = (j != k) ? mul(P, s_row[b][j]) : P;
+#pragma unroll
+            for (int a = 0; a < TR; a++) {
+                const int i = i0 + a;
+                if (i >= m) continue;
+                if (i == k) {
+                    reg[a][c] = pr;
+                } else if (j == k) {
+                    const B4<T> v = mul(s_col[b][i], P);
+                    reg[a][c] = B4<T>{-v.a, -v.b, -v.c, -v.d};
+                } else {
+                    mms(reg[a][c], s_col[b][i], pr);
+                }
+            }
+        }
     }
 #pragma unroll
-    for (int r = 0; r < R; r++)
-        if (ri[r] >= 0) store_b4(blk, SB + tid + r * bd, reg[r]);
+    for (int a = 0; a < TR; a++)
+#pragma unroll
+        for (int c = 0; c < TC; c++)
+            if (i0 + a < m && j0 + c < m) store_b4(blk, SB + (i0 + a) * m + j0 + c, reg[a][c]);
 }
 
-// Same, with S updated where it is stored (LDS or a global slice): for a
-// workgroup with no registers to spare.  Two barriers per step.
-template <typename T, int G, int MAXM>
+// Schur complement and inverse in one workgroup with no registers to spare:
+// S in registers, one block per thread, when the workgroup has ch_m^2
+// threads or more; otherwise S is updated where it is stored (LDS or a
+// global slice), two barriers per step.  Both do the same arithmetic.
+template <typename T, int G, int UNR, int MAXM>
 __device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad) {
     __shared__ B4<T> s_row[MAXM], s_col[MAXM];
     const int m = Y.ch_m, mm = m * m, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
-    chain_schur<T, G>(Y, blk);
+    chain_schur<T, G, UNR>(Y, blk, 0, bd / G);
     __syncthreads();
+    if (mm <= bd) {   // one block per thread fits: keep S in registers
+        chain_invert<T, 1, 1, MAXM>(Y, blk, bad);
+        return;
+    }
     for (int k = 0; k < m; k++) {
         for (int t = tid; t < m; t += bd) {
             s_row[t] = load_b4(blk, SB + k * m + t);

@@ -25,6 +25,9 @@ namespace pf {
 
 // longest trailing dense chain the single-network solver factors as one block
 constexpr int kChainMax = 101;
+// elimination-tree levels narrower than this run inside one workgroup (the
+// single-network solver's tail); the levels below it form subtree groups
+constexpr int kTailWidth = 64;
 
 // Ybus pattern (host, once per network): diagonal + both directions of every
 // branch, columns sorted; plus, per branch, the four CSR slots its pi-model

@@ -44,7 +44,6 @@ thread_local std::string g_last_error;
         if (s_ != PF_OK) return s_;         \
     } while (0)
 
-constexpr int kTailWidth = 16;   // levels this narrow run inside one workgroup
 
 }  // namespace
 
@@ -136,7 +135,8 @@ pf_status pf_solver::build() {
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&ctl_host, 2 * sizeof(Ctl), hipHostMallocDefault));
     yp = build_ypattern(net);
-    sym = analyze(net, yp, kTailWidth, opt.lu_blocked ? kChainMax : 0);
+    // lu_blocked = 0: one launch per elimination-tree level throughout
+    sym = analyze(net, yp, opt.lu_blocked ? kTailWidth : 0, opt.lu_blocked ? kChainMax : 0);
     if (!sym.level_ordered() || (sym.ch_m > 0 && sym.tail_start > sym.nlev_s)) {
         g_last_error = "symbolic analysis: nodes not numbered level by level, or chain outside the tail";
         return PF_INVALID_ARG;
@@ -305,15 +305,20 @@ template <typename T>
 pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     k_jfill<T><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk, gate);
     if (sym.n_parts > 0) {
-        k_factor_parts<T><<<sym.n_parts, 256, 0, stream>>>(dy, blk, ctl, gate);
+        k_factor_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, ctl, gate);
     } else {
         for (int l = 0; l < sym.tail_start; l++) {
             int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
             k_factor_level<T><<<(c + 3) / 4, 256, 0, stream>>>(dy, blk, ctl, sym.lvl_ptr[l], c, gate);
         }
     }
-    if (sym.tail_start < sym.nlev)   // narrow levels below the chain, then the chain
+    if (sym.tail_start < sym.nlev_s)   // narrow levels below the chain
         k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev_s, gate);
+    if (sym.ch_m > 0) {   // the dense chain: Schur complement across the GPU, then its inverse
+        const int ne = sym.fd_ptr[sym.n] - sym.fd_ptr[sym.ch_c0];
+        k_chain_schur<T><<<(ne + 15) / 16, 256, 0, stream>>>(dy, blk, ctl, gate);
+        k_chain_invert<T><<<1, ChainTiles<T>::threads, 0, stream>>>(dy, blk, ctl, gate);
+    }
     HIPCHK(hipGetLastError());
     return PF_OK;
 }
@@ -321,7 +326,7 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
 template <typename T>
 pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
     if (sym.n_parts > 0) {
-        k_fwd_parts<T><<<sym.n_parts, 256, 0, stream>>>(dy, blk, v, ctl, gate);
+        k_fwd_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, v, ctl, gate);
     } else {
         for (int l = 0; l < sym.tail_start; l++) {
             int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];
@@ -333,7 +338,7 @@ pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
     if (sym.tail_start < sym.nlev_s)
         k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
     if (sym.n_parts > 0) {
-        k_bwd_parts<T><<<sym.n_parts, 256, 0, stream>>>(dy, blk, v, ctl, gate);
+        k_bwd_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, v, ctl, gate);
     } else {
         for (int l = sym.tail_start - 1; l >= 0; l--) {
             int c = sym.lvl_ptr[l + 1] - sym.lvl_ptr[l];

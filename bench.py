@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--no-single", action="store_true", help="skip the single 15,104-bus solve")
     ap.add_argument("--single-only", action="store_true", help="only the single 15,104-bus solve")
     ap.add_argument("--single-variants", default="plain_fp64_eager,fp64_graph,mixed_graph")
-    ap.add_argument("--refine", type=int, default=2)
+    ap.add_argument("--refine", type=int, default=1, help="FP64 refinement steps after each FP32 solve")
     ap.add_argument("--no-warm", action="store_true", help="skip the learned warm-start legs")
     ap.add_argument("--warm-train", type=int, default=50000)
     ap.add_argument("--warm-eval", type=int, default=4096)

@@ -65,6 +65,9 @@ constexpr int kBatchChain = 32;
 // the whole workgroup on the level's destinations.  A constant, not the
 // number of lane groups, so that every workgroup size sums in the same order.
 constexpr int kWideLevel = 32;
+// update pairs loaded per round in the factorisation (registers are tight
+// in the persistent kernel)
+constexpr int kBatchUnroll = 1;
 
 // per-scenario working set, in LDS or in a global slice
 template <typename T>
@@ -123,10 +126,10 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
             const int d = d0 + t;
             B4<T> sa{0, 0, 0, 0}, su{0, 0, 0, 0};
             if (t == 0) {
-                pairs_acc<T, false, 2>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
+                pairs_acc<T, false, kBatchUnroll>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
                 a = load_b4(blk, j);
             } else {
-                pairs_acc<T, true, 2>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
+                pairs_acc<T, true, kBatchUnroll>(Y, blk, Y.fc_ptr[d], Y.fc_ptr[d + 1], 1, sa, su);
                 p = Y.colptr[j] + t - 1;
                 a = load_b4(blk, L0 + p);
                 u = load_b4(blk, U0 + p);
@@ -514,12 +517,12 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
                 } else {   // a run of narrow levels
                     int l1 = l + 1;
                     while (l1 < Y.nlev_s && Y.lvl_ptr[l1 + 1] - Y.lvl_ptr[l1] < kWideLevel) l1++;
-                    coop_factor_range<T, kCoopLanes, 2>(Y, w.blk, l, l1, &s_sing);
+                    coop_factor_range<T, kCoopLanes, kBatchUnroll>(Y, w.blk, l, l1, &s_sing);
                     l = l1;
                 }
             }
             if (Y.ch_m > 0) {
-                chain_factor_mem<T, GL, 2, kBatchChain>(Y, w.blk, &s_sing);
+                chain_factor_mem<T, GL, kBatchUnroll, kBatchChain>(Y, w.blk, &s_sing);
                 __syncthreads();
             }
         }

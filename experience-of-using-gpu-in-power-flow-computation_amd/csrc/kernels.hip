@@ -331,7 +331,76 @@ __global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    coop_factor_range<T, kCoopLanes, 4>(Y, blk, lv0, lv1, &ctl->singular);
+    if (Y.ck_lvl) chunk_factor_range<T, 4, 4, kMaxChunks>(Y, blk, lv0, lv1, &ctl->singular);
+    else coop_factor_range<T, kCoopLanes, 4>(Y, blk, lv0, lv1, &ctl->singular);
+}
+
+// Tail levels with the update pairs cut into chunks of at most
+// kChunkPairs, each inside one destination, so that one round of loads
+// covers a level whatever its longest destination:
+//   1. lane groups sum one chunk each (tree over the group) into LDS;
+//   2. one thread per destination adds its chunks in order and updates the
+//      block;
+//   3. one thread per destination inverts the pivot into the inverse-pivot
+//      region or scales its column block (both L copies).
+// Three barriers per level; the sums are in a fixed order.
+template <typename T, int G, int UNR, int MAXC>
+__device__ void chunk_factor_range(const DevSym &Y, T *blk, int lv0, int lv1, int *bad) {
+    __shared__ B4<T> s_pa[MAXC], s_pu[MAXC];
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / G, lane = tid % G, ng = bd / G;
+    for (int l = lv0; l < lv1; l++) {
+        const int c0 = Y.ck_lvl[l - Y.tail_start], nc = Y.ck_lvl[l - Y.tail_start + 1] - c0;
+        for (int c = grp; c < nc; c += ng) {
+            const int q0 = Y.ck_q0[c0 + c], q1f = Y.ck_q1[c0 + c], q1 = q1f & kChunkMask;
+            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+            if (q1f & kChunkDiag) pairs_acc<T, false, UNR>(Y, blk, q0 + lane, q1, G, a, u);
+            else pairs_acc<T, true, UNR>(Y, blk, q0 + lane, q1, G, a, u);
+            a = group_sum<T, G>(a);
+            u = group_sum<T, G>(u);
+            if (lane == 0) {
+                s_pa[c] = a;
+                s_pu[c] = u;
+            }
+        }
+        __syncthreads();
+        const int e0 = Y.ld_ptr[l] - Y.te0, ne = Y.ld_ptr[l + 1] - Y.ld_ptr[l];
+        for (int e = tid; e < ne; e += bd) {
+            const int E = e0 + e, j = Y.te_node[E], p = Y.te_p[E], cb = Y.te_ck[E], ce = Y.te_ck[E + 1];
+            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+            for (int c = cb; c < ce; c++) {
+                const B4<T> x = s_pa[c - c0], y = s_pu[c - c0];
+                a = B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
+                u = B4<T>{u.a + y.a, u.b + y.b, u.c + y.c, u.d + y.d};
+            }
+            if (p < 0) {
+                const B4<T> P = load_b4(blk, j);
+                store_b4(blk, j, B4<T>{P.a - a.a, P.b - a.b, P.c - a.c, P.d - a.d});
+            } else {
+                const B4<T> Lb = load_b4(blk, L0 + p), Ub = load_b4(blk, U0 + p);
+                store_b4(blk, L0 + p, B4<T>{Lb.a - a.a, Lb.b - a.b, Lb.c - a.c, Lb.d - a.d});
+                store_b4(blk, U0 + p, B4<T>{Ub.a - u.a, Ub.b - u.b, Ub.c - u.c, Ub.d - u.d});
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < ne; e += bd) {
+            const int E = e0 + e, j = Y.te_node[E], p = Y.te_p[E];
+            const B4<T> P = load_b4(blk, j);
+            T det = P.a * P.d - P.b * P.c;
+            const bool sing = !(det != (T)0) || !isfinite((double)det);
+            if (sing) det = (T)1;
+            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
+            if (p < 0) {
+                if (sing) atomicOr(bad, 1);
+                store_b4(blk, DI + j, Pi);
+            } else {
+                const B4<T> lb = mul(load_b4(blk, L0 + p), Pi);
+                store_b4(blk, L0 + p, lb);
+                store_b4(blk, LR0 + Y.lr_pos[p], lb);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // Schur complement of the trailing dense chain: every destination is

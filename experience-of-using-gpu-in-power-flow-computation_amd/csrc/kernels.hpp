@@ -71,6 +71,8 @@ constexpr bool chain_tiles_fit() {
 static_assert(chain_tiles_fit<float>() && chain_tiles_fit<double>(), "dense chain does not fit the registers");
 // lanes per destination block in the cooperative narrow-level factorisation
 constexpr int kCoopLanes = 4;
+// most chunks of update pairs on one tail level of the single-network solver
+constexpr int kMaxChunks = 1024;
 
 // Per-branch constants of the reported flows: the Ybus slots of (from, to)
 // and (to, from), and the reference's phif / bshf lookups in both directions.
@@ -94,6 +96,10 @@ struct DevSym {
     // groups of whole subtrees below the tail, one workgroup each
     int n_parts, part_levels;
     const int *part_ptr, *part_nodes;
+    // tail levels as chunks of update pairs (null: not chunked); te0 =
+    // ld_ptr[tail_start], the first flattened destination of the tail
+    const int *ck_lvl, *ck_q0, *ck_q1, *te_node, *te_p, *te_ck;
+    int te0, tail_start;
     const int *slot_e;
     const int *ld_ptr, *ld_node, *ld_d;     // flattened destinations per level
 };

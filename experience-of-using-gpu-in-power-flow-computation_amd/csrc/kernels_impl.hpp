@@ -307,6 +307,31 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
         }
         __syncthreads();
         const B4<T> P = pivot_inverse(s_row[b][k], bad);
+        if constexpr (sizeof(T) == 4) {   // registers to spare: read row and column once
+            B4<T> pr[TC], ci[TR];
+#pragma unroll
+            for (int c = 0; c < TC; c++) pr[c] = (j0 + c < m && j0 + c != k) ? mul(P, s_row[b][j0 + c]) : P;
+#pragma unroll
+            for (int a = 0; a < TR; a++) ci[a] = s_col[b][i0 + a < m ? i0 + a : 0];
+#pragma unroll
+            for (int a = 0; a < TR; a++) {
+                const int i = i0 + a;
+#pragma unroll
+                for (int c = 0; c < TC; c++) {
+                    const int j = j0 + c;
+                    if (i >= m || j >= m) continue;
+                    if (i == k) {
+                        reg[a][c] = pr[c];
+                    } else if (j == k) {
+                        const B4<T> v = mul(ci[a], P);
+                        reg[a][c] = B4<T>{-v.a, -v.b, -v.c, -v.d};
+                    } else {
+                        mms(reg[a][c], ci[a], pr[c]);
+                    }
+                }
+            }
+            continue;
+        }
         // column by column: one P S_kj live at a time, S_ik re-read from LDS
 #pragma unroll
         for (int c = 0; c < TC; c++) {

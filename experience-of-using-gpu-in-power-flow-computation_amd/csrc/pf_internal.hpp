@@ -82,6 +82,15 @@ struct Symbolic {
     // of level l
     int n_parts = 0;
     std::vector<int> part_ptr, part_nodes;
+    // the tail levels [tail_start, nlev_s) as chunks of at most kChunkPairs
+    // update pairs, each inside one destination: level tail_start + t has the
+    // chunks ck_lvl[t] .. ck_lvl[t+1]); chunk c covers pairs ck_q0[c] ..
+    // (ck_q1[c] & kChunkMask) of a diagonal destination when bit 30 is set.
+    // Destination E (a flattened ld entry, counted from the tail's first)
+    // belongs to node te_node[E], column slot te_p[E] (-1: the pivot) and
+    // owns the chunks te_ck[E] .. te_ck[E+1]).  ck_max: most chunks on a level.
+    std::vector<int> ck_lvl, ck_q0, ck_q1, te_node, te_p, te_ck;
+    int ck_max = 0;
     // Jacobian slot map: block slot -> Y entry (-1: fill-in, zero)
     //  slots [0, n): diagonal, [n, n+nnzL): L, [n+nnzL, n+2nnzL): U
     std::vector<int> slot_e;
@@ -102,6 +111,11 @@ struct Symbolic {
 
 // chain_cap: longest trailing dense chain factored as one block (0: none)
 Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap);
+
+constexpr int kChunkPairs = 16;
+constexpr int kChunkDiag = 1 << 30, kChunkMask = kChunkDiag - 1;
+// Builds the chunk lists of the tail levels [tail_start, nlev_s).
+void chunk_tail(Symbolic &s);
 
 // Splits the levels below s.tail_start into at most max_parts groups of
 // whole subtrees, balanced by factorisation work (longest processing time

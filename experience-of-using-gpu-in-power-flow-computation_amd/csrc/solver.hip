@@ -217,6 +217,19 @@ pf_status pf_solver::build() {
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
         partition_subtrees(sym, std::max(1, cus));
     }
+    chunk_tail(sym);
+    const bool chunked = sym.ck_max <= kMaxChunks && !sym.ck_q0.empty();
+    dy.tail_start = sym.tail_start;
+    dy.te0 = sym.ld_ptr[sym.tail_start];
+    dy.ck_lvl = dy.ck_q0 = dy.ck_q1 = dy.te_node = dy.te_p = dy.te_ck = nullptr;
+    if (chunked) {
+        PFCHK(upload((int **)&dy.ck_lvl, sym.ck_lvl));
+        PFCHK(upload((int **)&dy.ck_q0, sym.ck_q0));
+        PFCHK(upload((int **)&dy.ck_q1, sym.ck_q1));
+        PFCHK(upload((int **)&dy.te_node, sym.te_node));
+        PFCHK(upload((int **)&dy.te_p, sym.te_p));
+        PFCHK(upload((int **)&dy.te_ck, sym.te_ck));
+    }
     dy.n_parts = sym.n_parts;
     dy.part_levels = sym.tail_start;
     PFCHK(upload((int **)&dy.part_ptr, sym.part_ptr));

@@ -363,6 +363,32 @@ void partition_subtrees(Symbolic &s, int max_parts) {
     for (int j = 0; j < nw; j++) s.part_nodes[fill[(size_t)part[j] * T + s.level[j]]++] = j;
 }
 
+void chunk_tail(Symbolic &s) {
+    s.ck_lvl.assign(1, 0);
+    s.ck_q0.clear();
+    s.ck_q1.clear();
+    s.te_node.clear();
+    s.te_p.clear();
+    s.te_ck.clear();
+    s.ck_max = 0;
+    for (int l = s.tail_start; l < s.nlev_s; l++) {
+        const int before = (int)s.ck_q0.size();
+        for (int e = s.ld_ptr[l]; e < s.ld_ptr[l + 1]; e++) {
+            const int j = s.ld_node[e], d = s.ld_d[e], t = d - s.fd_ptr[j];
+            s.te_node.push_back(j);
+            s.te_p.push_back(t == 0 ? -1 : s.colptr[j] + t - 1);
+            s.te_ck.push_back((int)s.ck_q0.size());
+            for (int q = s.fc_ptr[d]; q < s.fc_ptr[d + 1]; q += kChunkPairs) {
+                s.ck_q0.push_back(q);
+                s.ck_q1.push_back(std::min(q + kChunkPairs, s.fc_ptr[d + 1]) | (t == 0 ? kChunkDiag : 0));
+            }
+        }
+        s.ck_max = std::max(s.ck_max, (int)s.ck_q0.size() - before);
+        s.ck_lvl.push_back((int)s.ck_q0.size());
+    }
+    s.te_ck.push_back((int)s.ck_q0.size());
+}
+
 // ------------------------------------------------------------- host numeric
 
 namespace {

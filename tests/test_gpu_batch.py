@@ -191,6 +191,7 @@ def test_batch_every_shape(pf, oracle, synth, cases, precision):
     Q = np.concatenate([A[1].ravel(), B[1].ravel()])
     L = np.concatenate([A[2].ravel(), B[2].ravel()])
     rows = []
+    answers = {}
     for lu_blocked in (0, 1):
         opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, lu_blocked=lu_blocked,
                           precision=pf.MIXED if precision == "mixed" else pf.FP64)
@@ -213,6 +214,7 @@ def test_batch_every_shape(pf, oracle, synth, cases, precision):
                         dv = max(dv, float(np.abs(V[off:off + nb] - r["V"]).max()))
                         dt = max(dt, float(np.abs(th[off:off + nb] - r["theta"]).max()))
                     off += nb
+                answers.setdefault(lu_blocked, []).append((V.copy(), th.copy()))
                 row = (lu_blocked, b.shape(0)["mode"], b.shape(1)["mode"], block, int(cv.sum()), dv, dt,
                        it.tolist(), [r["iterations"] for r in ref])
                 _say("lu_blocked %d modes %d/%d block %d converged %d dV %.3g dth %.3g iters %s oracle %s" % row)
@@ -220,3 +222,7 @@ def test_batch_every_shape(pf, oracle, synth, cases, precision):
     assert rows
     for row in rows:
         assert row[4] == len(ref) and row[5] <= 1e-6 and row[6] <= 1e-6, row
+    # every launch shape of one schedule gives the same answer bit for bit
+    for lb, sols in answers.items():
+        for V, th in sols[1:]:
+            assert np.array_equal(V, sols[0][0]) and np.array_equal(th, sols[0][1]), lb

@@ -123,6 +123,7 @@ def lib():
             L.pf_batch_set_scenarios.argtypes = [P, C.c_int, P, P, P, P, P, P]
             L.pf_batch_solve.argtypes = [P, P]
             L.pf_batch_get.argtypes = [P, P, P, P, P, P]
+            L.pf_batch_get_limq.argtypes = [P, P, P]
             L.pf_batch_autotune.argtypes = [P, C.c_char_p, C.c_int, P]
             L.pf_batch_get_shape.argtypes = [P, C.c_int, P, P, P, P]
             L.pf_batch_set_shape.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -428,6 +429,19 @@ class Batch:
         mm = np.zeros(self.n_scen)
         _chk(lib().pf_batch_get(self.h, _p(V), _p(th), _p(it), _p(cv), _p(mm)), "pf_batch_get")
         return V, th, it, cv.astype(bool), mm
+
+    def limq(self):
+        """Final PV->PQ state per scenario: (list of bool arrays [nPV of the
+        scenario's network], flips per scenario)."""
+        npv = [self.nets[i].nPV for i in self.net_of]
+        lq = np.zeros(max(1, sum(npv)), np.uint8)
+        fl = np.zeros(self.n_scen, np.int32)
+        _chk(lib().pf_batch_get_limq(self.h, _p(lq), _p(fl)), "pf_batch_get_limq")
+        out, o = [], 0
+        for k in npv:
+            out.append(lq[o:o + k].astype(bool))
+            o += k
+        return out, fl
 
 
 class WarmStart:

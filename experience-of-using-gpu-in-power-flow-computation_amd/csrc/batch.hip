@@ -50,6 +50,9 @@ struct BatchJob {
     double *Vout, *thout;               // packed results
     int *iters, *conv;
     double *maxmis;
+    const long long *pv_off;            // PV offset of scenario s in the packed limQ
+    unsigned char *limQout;             // packed final limQ [sum nPV]
+    int *flips;                         // PV<->PQ flips of the final solve, per scenario
     int *counter;                       // work queue head
     char *gws;                          // global workspace (per workgroup slice)
     long long ws_bytes;                 // bytes per workgroup slice
@@ -84,8 +87,8 @@ __host__ __device__ inline long long work_bytes(int nB, int n, int nPV, int nblk
     long long b = 0;
     b += 8LL * nB * 5;                 // V th P Q ql
     b += 8LL * 2 * n * 3;              // g x r
+    b = (b + 15) & ~15LL;              // E and I hold double2: 16-byte aligned
     b += 16LL * nB * 2;                // E I
-    b = (b + 15) & ~15LL;
     b += (long long)sizeof(T) * 4 * nblk;
     b += nB + nPV + 16;
     return (b + 255) & ~255LL;
@@ -104,9 +107,9 @@ __device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nblk) {
     w.g = (double *)take(16 * (size_t)n);
     w.x = (double *)take(16 * (size_t)n);
     w.r = (double *)take(16 * (size_t)n);
+    p = base + ((p - base + 15) & ~(size_t)15);   // 8 mod 16 when nB is odd
     w.E = (double2 *)take(16 * (size_t)nB);
     w.I = (double2 *)take(16 * (size_t)nB);
-    p = base + ((p - base + 15) & ~(size_t)15);
     w.blk = (T *)take(sizeof(T) * 4 * (size_t)nblk);
     w.isq = (unsigned char *)take(nB);
     w.limQ = (unsigned char *)take(nPV);
@@ -396,7 +399,7 @@ __device__ void wg_set_E(const DevNet &N, Work<T> &w) {
 template <typename T>
 __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const double *Pliq,
                       const double *Qliq, const double *Qload, const double *V0, const double *th0,
-                      int *s_it, int *s_conv, double *s_maxg) {
+                      int *s_it, int *s_conv, double *s_maxg, int *s_fl) {
     const DevNet &N = tp.N;
     const DevSym &Y = tp.Y;
     __shared__ unsigned long long s_maxG_bits, s_maxDx_bits;
@@ -566,6 +569,7 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
         *s_it = it;
         *s_conv = conv;
         *s_maxg = maxG;
+        *s_fl = s_flips;
     }
     __syncthreads();
 }
@@ -586,7 +590,7 @@ __host__ __device__ inline long long partial_lds_bytes(int n, int nlev) {
 template <typename T, int MODE, int MAXT>   // MAXT: largest workgroup (register budget)
 __global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
     extern __shared__ __align__(16) char smem[];
-    __shared__ int s_item, s_it, s_conv;
+    __shared__ int s_item, s_it, s_conv, s_fl;
     __shared__ double s_maxg;
     BatchTopo tp = tp0;
     const DevNet &N = tp.N;
@@ -623,20 +627,23 @@ __global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
         const double *Pl = J.Pliq + off, *Ql = J.Qliq + off, *Qd = J.Qload + off;
         bool warm = J.V0 != nullptr;
         wg_nr(tp, J, w, Pl, Ql, Qd, warm ? J.V0 + off : tp.V0flat, warm ? J.th0 + off : tp.th0flat,
-              &s_it, &s_conv, &s_maxg);
+              &s_it, &s_conv, &s_maxg, &s_fl);
         int total_it = s_it;
         if (warm && !s_conv) {                              // flat-start fallback
-            wg_nr(tp, J, w, Pl, Ql, Qd, tp.V0flat, tp.th0flat, &s_it, &s_conv, &s_maxg);
+            wg_nr(tp, J, w, Pl, Ql, Qd, tp.V0flat, tp.th0flat, &s_it, &s_conv, &s_maxg, &s_fl);
             total_it += s_it;
         }
         for (int k = threadIdx.x; k < N.nB; k += blockDim.x) {
             J.Vout[off + k] = w.V[k];
             J.thout[off + k] = w.th[k];
         }
+        const long long pvo = J.pv_off[sc];
+        for (int i = threadIdx.x; i < N.nPV; i += blockDim.x) J.limQout[pvo + i] = w.limQ[i];
         if (threadIdx.x == 0) {
             J.iters[sc] = total_it;
             J.conv[sc] = s_conv;
             J.maxmis[sc] = s_maxg;
+            J.flips[sc] = s_fl;
         }
         __syncthreads();
     }
@@ -718,13 +725,15 @@ struct pf_batch {
     hipGraphExec_t gexec = nullptr;
     int n_scen = 0;
     long long total = 0;
-    std::vector<long long> off;
+    std::vector<long long> off, pv_off;
     std::vector<int> net_of;
+    long long total_pv = 0;
     double *d_P = nullptr, *d_Q = nullptr, *d_L = nullptr, *d_V0 = nullptr, *d_T0 = nullptr;
     double *warm_V0 = nullptr, *warm_T0 = nullptr;
     double *d_V = nullptr, *d_T = nullptr, *d_mm = nullptr;
-    int *d_it = nullptr, *d_cv = nullptr, *d_counter = nullptr;
-    long long *d_off = nullptr;
+    int *d_it = nullptr, *d_cv = nullptr, *d_counter = nullptr, *d_fl = nullptr;
+    long long *d_off = nullptr, *d_pvoff = nullptr;
+    unsigned char *d_lq = nullptr;
     int cus = 0;
 
     template <typename T>
@@ -917,6 +926,7 @@ pf_status pf_batch::launch_net(size_t i, hipStream_t s) {
         J.V0 = d_V0; J.th0 = d_T0;
         J.Vout = d_V; J.thout = d_T;
         J.iters = d_it; J.conv = d_cv; J.maxmis = d_mm;
+        J.pv_off = d_pvoff; J.limQout = d_lq; J.flips = d_fl;
         J.counter = d_counter + i;
         J.gws = b.d_ws;
         J.ws_bytes = b.ws;
@@ -997,7 +1007,8 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
         b->n_scen = n_scen;
         b->net_of.assign(net_of, net_of + n_scen);
         b->off.resize(n_scen);
-        long long tot = 0;
+        b->pv_off.resize(n_scen);
+        long long tot = 0, tot_pv = 0;
         for (auto &nb : b->nets) nb->scen.clear();
         for (int s = 0; s < n_scen; s++) {
             if (net_of[s] < 0 || net_of[s] >= (int)b->nets.size()) {
@@ -1005,10 +1016,13 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
                 return PF_INVALID_ARG;
             }
             b->off[s] = tot;
+            b->pv_off[s] = tot_pv;
             tot += b->nets[net_of[s]]->net.nB;
+            tot_pv += b->nets[net_of[s]]->net.nPV;
             b->nets[net_of[s]]->scen.push_back(s);
         }
         b->total = tot;
+        b->total_pv = tot_pv;
         BCHK(b->alloc(&b->d_P, tot));
         BCHK(b->alloc(&b->d_Q, tot));
         BCHK(b->alloc(&b->d_L, tot));
@@ -1017,7 +1031,10 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
         BCHK(b->alloc(&b->d_mm, n_scen));
         BCHK(b->alloc(&b->d_it, n_scen));
         BCHK(b->alloc(&b->d_cv, n_scen));
+        BCHK(b->alloc(&b->d_fl, n_scen));
+        BCHK(b->alloc(&b->d_lq, tot_pv));
         BCHK(b->upload(&b->d_off, b->off));
+        BCHK(b->upload(&b->d_pvoff, b->pv_off));
         for (auto &nbp : b->nets) {
             BatchNet &nb = *nbp;
             if (nb.scen.empty()) continue;
@@ -1220,6 +1237,16 @@ pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations, i
     if (iterations) BHIP(hipMemcpy(iterations, b->d_it, sizeof(int) * b->n_scen, hipMemcpyDeviceToHost));
     if (converged) BHIP(hipMemcpy(converged, b->d_cv, sizeof(int) * b->n_scen, hipMemcpyDeviceToHost));
     if (max_mismatch) BHIP(hipMemcpy(max_mismatch, b->d_mm, sizeof(double) * b->n_scen, hipMemcpyDeviceToHost));
+    return PF_OK;
+}
+
+pf_status pf_batch_get_limq(pf_batch *b, unsigned char *limQ, int *flips) {
+    if (!b || !b->n_scen) return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    BHIP(hipStreamSynchronize(b->st[0]));
+    if (limQ && b->total_pv > 0)
+        BHIP(hipMemcpy(limQ, b->d_lq, (size_t)b->total_pv, hipMemcpyDeviceToHost));
+    if (flips) BHIP(hipMemcpy(flips, b->d_fl, sizeof(int) * b->n_scen, hipMemcpyDeviceToHost));
     return PF_OK;
 }
 

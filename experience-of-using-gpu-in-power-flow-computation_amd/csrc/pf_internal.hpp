@@ -28,6 +28,8 @@ constexpr int kChainMax = 101;
 // elimination-tree levels narrower than this run inside one workgroup (the
 // single-network solver's tail); the levels below it form subtree groups
 constexpr int kTailWidth = 64;
+// the same threshold for the straightforward level schedule (lu_blocked = 0)
+constexpr int kPlainTailWidth = 16;
 
 // Ybus pattern (host, once per network): diagonal + both directions of every
 // branch, columns sorted; plus, per branch, the four CSR slots its pi-model

@@ -135,8 +135,10 @@ pf_status pf_solver::build() {
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void **)&ctl_host, 2 * sizeof(Ctl), hipHostMallocDefault));
     yp = build_ypattern(net);
-    // lu_blocked = 0: one launch per elimination-tree level throughout
-    sym = analyze(net, yp, opt.lu_blocked ? kTailWidth : 0, opt.lu_blocked ? kChainMax : 0);
+    // lu_blocked = 0, the straightforward level schedule: one launch per
+    // elimination-tree level, levels narrower than kPlainTailWidth nodes in
+    // one workgroup, no dense chain and no subtree groups
+    sym = analyze(net, yp, opt.lu_blocked ? kTailWidth : kPlainTailWidth, opt.lu_blocked ? kChainMax : 0);
     if (!sym.level_ordered() || (sym.ch_m > 0 && sym.tail_start > sym.nlev_s)) {
         g_last_error = "symbolic analysis: nodes not numbered level by level, or chain outside the tail";
         return PF_INVALID_ARG;

@@ -232,6 +232,11 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of,
 pf_status pf_batch_solve(pf_batch *b, double *elapsed_ms);
 pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations,
                        int *converged, double *max_mismatch);
+/* The PV->PQ decision state at the end of each scenario's solve: limQ packed
+ * per scenario (nPV of its network, scenarios in order; the reference's
+ * iterativo->limQ, CPU/limInjReat.cpp:38-91) and the PV<->PQ flips of that
+ * solve (flat-start fallback: of the fallback solve).  Either may be NULL.  */
+pf_status pf_batch_get_limq(pf_batch *b, unsigned char *limQ, int *flips);
 /* Autotuner (no counterpart in the reference): for every topology of the
  * current scenario set, times one launch of its persistent kernel for each
  * candidate launch shape (working set fully in LDS / step vectors and level

@@ -55,7 +55,7 @@ def lib():
         L.orc_min_degree.argtypes = [C.c_int, ip, ip, ip]
         L.orc_limits.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.orc_nr.argtypes = [P, C.POINTER(OrcOpts), P, P, P, P, P, P, C.POINTER(OrcResult)]
-        L.orc_nr_batch.argtypes = [P, C.POINTER(OrcOpts), C.c_int, P, P, P, P, P, P, P, C.c_int]
+        L.orc_nr_batch.argtypes = [P, C.POINTER(OrcOpts), C.c_int, P, P, P, P, P, P, P, P, C.c_int, C.c_int]
         L.orc_flows.argtypes = [P, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -135,16 +135,21 @@ class Case:
         return tuple(out)
 
     def solve_batch(self, Pliq, Qliq, Qload, tol=1e-8, max_iter=100, lim=True,
-                    reuse_symbolic=False, nthreads=1):
+                    reuse_symbolic=False, nthreads=1, inner=False, want_limq=False):
+        """nthreads scenario-parallel threads, or (inner=True) scenarios one
+        after another with nthreads OpenMP threads inside each solve."""
         nsc = Pliq.shape[0]
         o = OrcOpts(tol, max_iter, int(lim), 0, int(reuse_symbolic))
         V = np.zeros((nsc, self.nB))
         th = np.zeros((nsc, self.nB))
         it = np.zeros(nsc, np.int32)
         cv = np.zeros(nsc, np.int32)
+        lq = np.zeros((nsc, max(self.nPV, 1)), np.uint8)
         P_, Q_, L_ = (np.ascontiguousarray(a, np.float64) for a in (Pliq, Qliq, Qload))
         lib().orc_nr_batch(self.h, C.byref(o), nsc, _ptr(P_), _ptr(Q_), _ptr(L_), _ptr(V),
-                           _ptr(th), _ptr(it), _ptr(cv), nthreads)
+                           _ptr(th), _ptr(it), _ptr(cv), _ptr(lq), nthreads, int(inner))
+        if want_limq:
+            return V, th, it, cv.astype(bool), lq[:, :self.nPV].astype(bool)
         return V, th, it, cv.astype(bool)
 
 

@@ -1029,19 +1029,39 @@ int orc_nr(const orc_case *c, const orc_opts *o, const double *Pliq_in,
 
 int orc_nr_batch(const orc_case *c, const orc_opts *o, int nsc, const double *Pliq,
                  const double *Qliq, const double *Qload, double *V, double *theta,
-                 int *iters, int *converged, int nthreads) {
+                 int *iters, int *converged, unsigned char *limQ, int nthreads, int inner) {
+    int nB = c->nB, nPV = c->nPV;
+    int nt = nthreads > 0 ? nthreads : 1;
+    if (inner) {
+        /* the reference's own shape: one case at a time, OpenMP inside the
+           solve on the loops it parallelises (CPU/esparso/potencia/spCalcPQliq.cpp) */
+        orc_opts oi = *o;
+        oi.openmp = nt > 1;
+#ifdef _OPENMP
+        omp_set_num_threads(nt);
+#endif
+        for (int s = 0; s < nsc; s++) {
+            orc_result r;
+            double *Vs = V + (size_t)s * nB, *Ts = theta + (size_t)s * nB;
+            memcpy(Vs, c->V, sizeof(double) * nB);
+            memcpy(Ts, c->theta, sizeof(double) * nB);
+            orc_nr(c, &oi, Pliq + (size_t)s * nB, Qliq + (size_t)s * nB, Qload + (size_t)s * nB, Vs, Ts,
+                   limQ ? limQ + (size_t)s * nPV : NULL, &r);
+            if (iters) iters[s] = r.iterations;
+            if (converged) converged[s] = r.converged;
+        }
+        return 0;
+    }
     orc_opts oi = *o;
     oi.openmp = 0;
-    int nB = c->nB;
-    (void)nthreads;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
     for (int s = 0; s < nsc; s++) {
         orc_result r;
         double *Vs = V + (size_t)s * nB, *Ts = theta + (size_t)s * nB;
         memcpy(Vs, c->V, sizeof(double) * nB);
         memcpy(Ts, c->theta, sizeof(double) * nB);
         orc_nr(c, &oi, Pliq + (size_t)s * nB, Qliq + (size_t)s * nB, Qload + (size_t)s * nB,
-               Vs, Ts, NULL, &r);
+               Vs, Ts, limQ ? limQ + (size_t)s * nPV : NULL, &r);
         if (iters) iters[s] = r.iterations;
         if (converged) converged[s] = r.converged;
     }

@@ -102,9 +102,12 @@ int  orc_nr(const orc_case *c, const orc_opts *o, const double *Pliq,
 void orc_flows(const orc_case *c, const double *V, const double *theta, double *Pdp, double *Qdp,
                double *Ppd, double *Qpd);
 
+/* limQ [nsc x nPV] may be NULL.  inner = 0: scenario-parallel over nthreads  */
+/* threads; inner = 1: scenarios one after another, OpenMP over nthreads     */
+/* threads inside each solve (the reference's shape).                        */
 int  orc_nr_batch(const orc_case *c, const orc_opts *o, int nsc, const double *Pliq,
                   const double *Qliq, const double *Qload, double *V, double *theta,
-                  int *iters, int *converged, int nthreads);
+                  int *iters, int *converged, unsigned char *limQ, int nthreads, int inner);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,7 @@ def test_batch_uneven(pf, oracle, synth, cases, precision):
     b.set_scenarios(net_of, np.concatenate(P), np.concatenate(Q), np.concatenate(L))
     ms = b.solve()
     V, th, it, cv, mm = b.results()
+    lq, fl = b.limq()
     _say("elapsed ms", ms, "iters", it.tolist())
     off = 0
     for s, (k, i) in enumerate(picked):
@@ -72,8 +73,10 @@ def test_batch_uneven(pf, oracle, synth, cases, precision):
         assert mm[s] <= 1e-8
         assert np.abs(V[off:off + nb] - r["V"]).max() <= 1e-6
         assert np.abs(th[off:off + nb] - r["theta"]).max() <= 1e-6
+        assert np.array_equal(lq[s], r["limQ"]), (s, k, i)
         if precision == "fp64":
             assert it[s] == r["iterations"]
+            assert fl[s] == r["n_switch_events"]
         off += nb
     # replaying the captured graph gives the same answer
     b.solve()
@@ -81,7 +84,46 @@ def test_batch_uneven(pf, oracle, synth, cases, precision):
     assert np.array_equal(V, V2) and np.array_equal(it, it2)
 
 
+def test_batch_warm_start_flat_parity_without_limits(pf, oracle, synth, cases):
+    """Without Q limits the power-flow answer does not depend on the start:
+    every warm-started scenario must reproduce the FLAT-start oracle (the
+    reference path, CPU/estrutura/barra.cpp:20-25) within 1e-6 pu / 1e-6 rad,
+    in fewer iterations."""
+    for name, n in (("c118", 16), ("c944", 6)):
+        p = os.path.join(GOLDEN, "c118.cdf") if name == "c118" else cases.path(name)
+        net = pf.Network(p)
+        oc = oracle.Case(p)
+        P, Q, L = _scenarios(synth, net, n, 3250000)
+        ref = _oracle_solve(oc, P, Q, L, lim=False)
+        opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=0, precision=pf.MIXED)
+        b = pf.Batch([net], opts)
+        net_of = np.zeros(n, np.int32)
+        b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel())
+        b.solve()
+        _, _, it_flat, cv_flat, _ = b.results()
+        rng = np.random.default_rng(5)
+        V0 = np.concatenate([r["V"] for r in ref]) + 5e-3 * rng.standard_normal(n * net.nB)
+        T0 = np.concatenate([r["theta"] for r in ref]) + 5e-3 * rng.standard_normal(n * net.nB)
+        b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel(), V0, T0)
+        b.solve()
+        V, th, it, cv, mm = b.results()
+        nb = net.nB
+        dv = max(float(np.abs(V[nb * s:nb * (s + 1)] - ref[s]["V"]).max()) for s in range(n))
+        dt = max(float(np.abs(th[nb * s:nb * (s + 1)] - ref[s]["theta"]).max()) for s in range(n))
+        _say("warm (no limits)", name, "iters flat", it_flat.tolist(), "warm", it.tolist(), "dV", dv, "dth", dt)
+        assert cv_flat.all() and cv.all() and (mm <= 1e-8).all()
+        assert all(r["converged"] for r in ref)
+        assert dv <= 1e-6 and dt <= 1e-6
+        assert it.mean() < it_flat.mean()
+
+
 def test_batch_warm_start_and_fallback(pf, oracle, synth):
+    """With Q limits on, the answer depends on the switching path: a bus
+    switched back to PV keeps the |V| it reached while limited
+    (CPU/limInjReat.cpp:62-87), and every scenario here switches back at
+    least once.  So a warm-started solve is checked against the oracle run
+    from the same start (the same rules applied along the same path), and
+    the flat-start fallback against the flat-start oracle."""
     p118 = os.path.join(GOLDEN, "c118.cdf")
     net = pf.Network(p118)
     oc = oracle.Case(p118)
@@ -105,6 +147,7 @@ def test_batch_warm_start_and_fallback(pf, oracle, synth):
     b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel(), V0, T0)
     b.solve()
     V, th, it, cv, mm = b.results()
+    lq_w, _ = b.limq()
     assert cv.all() and (mm <= 1e-8).all()
     assert it.mean() < it_flat.mean()
     is_pq = np.zeros(118, bool)
@@ -118,15 +161,20 @@ def test_batch_warm_start_and_fallback(pf, oracle, synth):
         assert r["converged"]
         assert np.abs(V[118 * s:118 * (s + 1)] - r["V"]).max() <= 1e-6
         assert np.abs(th[118 * s:118 * (s + 1)] - r["theta"]).max() <= 1e-6
-    # a useless start (|V| = 0.05, random angles) diverges and falls back to flat start
+        assert np.array_equal(lq_w[s], r["limQ"])
+    # a useless start (|V| = 0.05, random angles) diverges and falls back to
+    # flat start: then the answer and the decisions are the flat-start oracle's
     bad_V = np.full_like(V0, 0.05)
     bad_T = np.random.default_rng(1).uniform(-3, 3, T0.shape)
     b.set_scenarios(net_of, P.ravel(), Q.ravel(), L.ravel(), bad_V, bad_T)
     b.solve()
     V, th, it, cv, mm = b.results()
+    lq, fl = b.limq()
     assert cv.all()
     for s in range(8):
         assert np.abs(V[118 * s:118 * (s + 1)] - ref[s]["V"]).max() <= 1e-6
+        assert np.abs(th[118 * s:118 * (s + 1)] - ref[s]["theta"]).max() <= 1e-6
+        assert np.array_equal(lq[s], ref[s]["limQ"])
 
 
 def test_batch_autotune(pf, synth, cases, tmp_path):
@@ -206,23 +254,70 @@ def test_batch_every_shape(pf, oracle, synth, cases, precision):
                     continue
                 b.solve()
                 V, th, it, cv, mm = b.results()
+                lq, _ = b.limq()
                 dv = dt = 0.0
                 off = 0
+                same_lq = True
                 for s, r in enumerate(ref):
                     nb = (n118, n944)[net_of[s]].nB
                     if cv[s] and r["converged"]:
                         dv = max(dv, float(np.abs(V[off:off + nb] - r["V"]).max()))
                         dt = max(dt, float(np.abs(th[off:off + nb] - r["theta"]).max()))
+                    same_lq &= bool(np.array_equal(lq[s], r["limQ"]))
                     off += nb
                 answers.setdefault(lu_blocked, []).append((V.copy(), th.copy()))
                 row = (lu_blocked, b.shape(0)["mode"], b.shape(1)["mode"], block, int(cv.sum()), dv, dt,
-                       it.tolist(), [r["iterations"] for r in ref])
-                _say("lu_blocked %d modes %d/%d block %d converged %d dV %.3g dth %.3g iters %s oracle %s" % row)
+                       it.tolist(), [r["iterations"] for r in ref], same_lq)
+                _say("lu_blocked %d modes %d/%d block %d converged %d dV %.3g dth %.3g iters %s oracle %s "
+                     "same limQ %s" % row)
                 rows.append(row)
     assert rows
     for row in rows:
-        assert row[4] == len(ref) and row[5] <= 1e-6 and row[6] <= 1e-6, row
+        assert row[4] == len(ref) and row[5] <= 1e-6 and row[6] <= 1e-6 and row[9], row
     # every launch shape of one schedule gives the same answer bit for bit
     for lb, sols in answers.items():
         for V, th in sols[1:]:
             assert np.array_equal(V, sols[0][0]) and np.array_equal(th, sols[0][1]), lb
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_1888(pf, oracle, synth, cases, precision):
+    """The 1,888-bus (16-copy) scenarios of the uneven benchmark batch, in the
+    launch shape the autotuner picks for them (step vectors and level data in
+    LDS, the factor in a global slice, 1024 threads), against the oracle:
+    V / theta within 1e-6, the same final PV->PQ set, and in FP64 the same
+    iteration and switch counts."""
+    p = cases.path("c1888")
+    net = pf.Network(p)
+    oc = oracle.Case(p)
+    n = 10
+    P, Q, L = _scenarios(synth, net, n, 3000000 + 2048)
+    ref = _oracle_solve(oc, P, Q, L)
+    opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1,
+                      precision=pf.MIXED if precision == "mixed" else pf.FP64)
+    b = pf.Batch([net], opts)
+    b.set_scenarios(np.zeros(n, np.int32), P.ravel(), Q.ravel(), L.ravel())
+    b.autotune(None, reps=1)
+    shape = b.shape(0)
+    _say("1888", precision, "shape", shape)
+    b.solve()
+    V, th, it, cv, mm = b.results()
+    lq, fl = b.limq()
+    nb = net.nB
+    rows = []
+    for s in range(n):
+        r = ref[s]
+        dv = float(np.abs(V[nb * s:nb * (s + 1)] - r["V"]).max())
+        dt = float(np.abs(th[nb * s:nb * (s + 1)] - r["theta"]).max())
+        rows.append((s, bool(cv[s]), r["converged"], int(it[s]), r["iterations"], int(fl[s]),
+                     r["n_switch_events"], dv, dt, bool(np.array_equal(lq[s], r["limQ"]))))
+        _say("scenario %d conv %s/%s iters %d/%d flips %d/%d dV %.3g dth %.3g same limQ %s" % rows[-1])
+    assert shape["mode"] == 1
+    assert sum(r[2] for r in rows) >= n - 2
+    for row in rows:
+        assert row[1] == row[2], row
+        assert row[9], row
+        if row[2]:
+            assert row[7] <= 1e-6 and row[8] <= 1e-6, row
+        if precision == "fp64":
+            assert row[3] == row[4] and row[5] == row[6], row

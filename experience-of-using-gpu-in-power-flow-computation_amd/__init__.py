@@ -145,7 +145,7 @@ def _p(a):
 
 def _chk(s: int, what: str):
     if s != PF_OK:
-        raise PfError(s, what + (f" [{last_error()}]" if s == PF_DEVICE_ERROR else ""))
+        raise PfError(s, what + (f" [{last_error()}]" if s in (PF_DEVICE_ERROR, PF_INVALID_ARG) else ""))
 
 
 def options(text: str | None = None, cpu_layout: int = 0, **kw) -> Options:

@@ -119,7 +119,7 @@ pf_status pf_probe_create(const pf_network *net, pf_symbolic_probe **out, int *n
     if (!net || !out || net->nB < 2) return PF_INVALID_ARG;
     auto p = std::make_unique<pf_symbolic_probe>();
     p->yp = pf::build_ypattern(*net);
-    p->sym = pf::analyze(*net, p->yp, pf::kTailWidth, pf::kChainMax);   // the single-network solver's analysis
+    p->sym = pf::analyze(*net, p->yp, pf::kTailWidth, pf::kChainMaxF64, pf::kTopWidth);   // the single-network solver's analysis
     if (!p->sym.level_ordered()) return PF_INVALID_ARG;
     pf::partition_subtrees(p->sym, 256);
     if (n) *n = p->sym.n;

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,7 @@ struct BatchTopo {
 
 struct BatchJob {
     int nscen;                          // scenarios of this topology
+    const int *nscen_dev;               // when set: the count, read on the device
     const int *scen;                    // global scenario index
     const long long *off;               // bus offset of scenario s in the packed arrays
     const double *Pliq, *Qliq, *Qload;  // packed [sum nB]
@@ -61,9 +63,14 @@ struct BatchJob {
 };
 
 constexpr int GL = 16;   // lanes per node in the in-workgroup LU and solves
-// trailing dense chain: at most kBatchChain nodes (inverted where the
-// working set lives: the registers are taken by the rest of the solve)
+// dense top block: at most kBatchChain nodes when the working set sits in
+// LDS (MODE 2), kBatchChainWide otherwise; S is inverted in register tiles
+// of up to kBatchTile<T> x kBatchTile<T> blocks per thread, or where the
+// working set lives when the workgroup cannot cover it
 constexpr int kBatchChain = 32;
+constexpr int kBatchChainWide = 64;
+template <typename T> constexpr int kBatchTile = 2;   // wider tiles spill in the persistent kernel
+template <int MODE> constexpr int batch_chain_cap() { return MODE == 2 ? kBatchChain : kBatchChainWide; }
 // levels of at least kWideLevel nodes: a lane group per node; narrower ones:
 // the whole workgroup on the level's destinations.  A constant, not the
 // number of lane groups, so that every workgroup size sums in the same order.
@@ -171,7 +178,7 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
 // previous level's arithmetic and barrier; the summation order is the same
 // as without it.
 template <typename T>
-__device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
+__device__ void wg_solve(const DevSym &Y, const T *blk, double *x, double *s_r) {
     const int grp = (int)threadIdx.x / GL, lane = (int)threadIdx.x % GL, ng = (int)blockDim.x / GL;
     const int LR0 = Y.n + 2 * Y.nnzL, U0 = Y.n + Y.nnzL;
     const int nl = Y.nlev_s;
@@ -240,7 +247,7 @@ __device__ void wg_solve(const DevSym &Y, const T *blk, double *x) {
     int pr = -1;
     B4<T> pU{0, 0, 0, 0}, pD{0, 0, 0, 0};
     if (nl > 0) fetch_bwd(nl - 1, pr, pU, pD);
-    if (Y.ch_m > 0) chain_solve<T, GL, kBatchChain>(Y, blk, x);
+    if (Y.ch_m > 0) chain_solve<T, GL>(Y, blk, x, s_r);
     for (int l = nl - 1; l >= 0; l--) {
         int nr = -1;
         B4<T> nU{0, 0, 0, 0}, nD{0, 0, 0, 0};
@@ -399,7 +406,7 @@ __device__ void wg_set_E(const DevNet &N, Work<T> &w) {
 template <typename T>
 __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const double *Pliq,
                       const double *Qliq, const double *Qload, const double *V0, const double *th0,
-                      int *s_it, int *s_conv, double *s_maxg, int *s_fl) {
+                      int *s_it, int *s_conv, double *s_maxg, int *s_fl, B4<T> *s_chain) {
     const DevNet &N = tp.N;
     const DevSym &Y = tp.Y;
     __shared__ unsigned long long s_maxG_bits, s_maxDx_bits;
@@ -525,16 +532,16 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
                 }
             }
             if (Y.ch_m > 0) {
-                chain_factor_mem<T, GL, kBatchUnroll, kBatchChain>(Y, w.blk, &s_sing);
+                chain_factor_mem<T, GL, kBatchUnroll, kBatchTile<T>>(Y, w.blk, &s_sing, s_chain);
                 __syncthreads();
             }
         }
         if (s_sing) break;
-        wg_solve(Y, (const T *)w.blk, w.x);
+        wg_solve(Y, (const T *)w.blk, w.x, (double *)s_chain);
         for (int rs = 0; rs < J.refine; rs++) {
             wg_residual(N, w);
             __syncthreads();
-            wg_solve(Y, (const T *)w.blk, w.r);
+            wg_solve(Y, (const T *)w.blk, w.r, (double *)s_chain);
             for (int i = tid; i < 2 * Y.n; i += blockDim.x) w.x[i] += w.r[i];
             __syncthreads();
         }
@@ -591,6 +598,7 @@ template <typename T, int MODE, int MAXT>   // MAXT: largest workgroup (register
 __global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_item, s_it, s_conv, s_fl;
+    __shared__ B4<T> s_chain[4 * batch_chain_cap<MODE>()];   // dense top: row / column staging
     __shared__ double s_maxg;
     BatchTopo tp = tp0;
     const DevNet &N = tp.N;
@@ -616,21 +624,22 @@ __global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
         tp.Y.fd_ptr = stage(tp0.Y.fd_ptr, n + 1);
         __syncthreads();
     }
+    const int nscen = J.nscen_dev ? *J.nscen_dev : J.nscen;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(J.counter, 1);
         __syncthreads();
         const int item = s_item;
         __syncthreads();
-        if (item >= J.nscen) break;                         // every wave reaches this exit
+        if (item >= nscen) break;                           // every wave reaches this exit
         const int sc = J.scen[item];
         const long long off = J.off[sc];
         const double *Pl = J.Pliq + off, *Ql = J.Qliq + off, *Qd = J.Qload + off;
         bool warm = J.V0 != nullptr;
         wg_nr(tp, J, w, Pl, Ql, Qd, warm ? J.V0 + off : tp.V0flat, warm ? J.th0 + off : tp.th0flat,
-              &s_it, &s_conv, &s_maxg, &s_fl);
+              &s_it, &s_conv, &s_maxg, &s_fl, s_chain);
         int total_it = s_it;
         if (warm && !s_conv) {                              // flat-start fallback
-            wg_nr(tp, J, w, Pl, Ql, Qd, tp.V0flat, tp.th0flat, &s_it, &s_conv, &s_maxg, &s_fl);
+            wg_nr(tp, J, w, Pl, Ql, Qd, tp.V0flat, tp.th0flat, &s_it, &s_conv, &s_maxg, &s_fl, s_chain);
             total_it += s_it;
         }
         for (int k = threadIdx.x; k < N.nB; k += blockDim.x) {
@@ -647,6 +656,84 @@ __global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
         }
         __syncthreads();
     }
+}
+
+
+// ------------------------------------------------------------ fixed scratch
+// The allocation-free batch entry point keeps its per-call tables (packed
+// offsets, per-network scenario lists, work counters) and the per-workgroup
+// working-set slices in this fixed array, one copy per GPU, held until the
+// process ends.  Host code must use the address hipGetSymbolAddress returns
+// for the current device (noalloc_scratch), never the array's name.
+constexpr long long kScratchBytes = 64LL << 20;
+constexpr int kScratchHeader = 1024;        // counts [64], counters [64], invalid flag
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+struct NoallocNets {
+    int n_nets;
+    int nB[64], nPV[64];
+};
+
+// exclusive prefix of v over the workgroup (1024 threads at most); *total:
+// the sum over all threads
+__device__ long long block_scan_excl(long long v, long long *total) {
+    __shared__ long long buf[1024];
+    const int t = threadIdx.x;
+    buf[t] = v;
+    __syncthreads();
+    for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+        const long long a = t >= d ? buf[t - d] : 0;
+        __syncthreads();
+        buf[t] += a;
+        __syncthreads();
+    }
+    const long long incl = buf[t];
+    *total = buf[blockDim.x - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// One workgroup: the packed bus / PV offsets of every scenario, the list of
+// scenarios of network i (in order) at scen + i * n_scen with its length in
+// count[i], the work counters cleared, and a flag for net_of entries outside
+// [0, n_nets) (those scenarios are left unsolved).
+__global__ void __launch_bounds__(1024) k_noalloc_tables(const int *net_of, int n_scen, NoallocNets nn,
+                                                         long long *off, long long *pvoff, int *scen,
+                                                         int *count, int *counter, int *invalid) {
+    const int T = blockDim.x, t = threadIdx.x;
+    const int per = (n_scen + T - 1) / T;
+    const int s0 = min(n_scen, t * per), s1 = min(n_scen, s0 + per);
+    long long a = 0, b = 0;
+    int bad = 0;
+    for (int s = s0; s < s1; s++) {
+        const int k = net_of[s];
+        if (k < 0 || k >= nn.n_nets) { bad = 1; continue; }
+        a += nn.nB[k];
+        b += nn.nPV[k];
+    }
+    long long ta, tb;
+    long long ea = block_scan_excl(a, &ta), eb = block_scan_excl(b, &tb);
+    for (int s = s0; s < s1; s++) {
+        const int k = net_of[s];
+        off[s] = ea;
+        pvoff[s] = eb;
+        if (k < 0 || k >= nn.n_nets) continue;
+        ea += nn.nB[k];
+        eb += nn.nPV[k];
+    }
+    bad = __syncthreads_or(bad);
+    for (int i = 0; i < nn.n_nets; i++) {
+        long long c = 0, tot;
+        for (int s = s0; s < s1; s++) c += net_of[s] == i;
+        long long e = block_scan_excl(c, &tot);
+        for (int s = s0; s < s1; s++)
+            if (net_of[s] == i) scen[(long long)i * n_scen + e++] = s;
+        if (t == 0) {
+            count[i] = (int)tot;
+            counter[i] = 0;
+        }
+    }
+    if (t == 0) *invalid = bad;
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -677,12 +764,11 @@ inline BatchKernel batch_kernel(bool mixed, int mode, int block) {
 // ======================================================================== host
 
 namespace {
-thread_local std::string g_batch_error;
 #define BHIP(expr)                                                                 \
     do {                                                                           \
         hipError_t e_ = (expr);                                                    \
         if (e_ != hipSuccess) {                                                    \
-            g_batch_error = std::string(#expr) + ": " + hipGetErrorString(e_);     \
+            pf::set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));     \
             return PF_DEVICE_ERROR;                                                \
         }                                                                          \
     } while (0)
@@ -692,6 +778,23 @@ thread_local std::string g_batch_error;
         if (s_ != PF_OK) return s_; \
     } while (0)
 constexpr long long kLdsLimit = 64 * 1024;
+}  // namespace
+
+namespace {
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[64] = {};   // address of `scratch` on each device
+
+// once per device, with that device current: the device address of the
+// fixed scratch array
+pf_status noalloc_setup(int device) {
+    if (device < 0 || device >= 64) return PF_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PF_OK;
+    void *p = nullptr;
+    BHIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = (char *)p;
+    return PF_OK;
+}
 }  // namespace
 
 struct BatchNet {
@@ -779,9 +882,16 @@ pf_status pf_batch::add_net(const pf_network &src) {
     const long long max_lds = max_lds_i - 8192;   // room for the kernel's static LDS
     b->net = src;
     b->yp = build_ypattern(src);
-    b->sym = analyze(src, b->yp, 0, opt.lu_blocked ? kBatchChain : 0);
+    b->sym = analyze(src, b->yp, 0, opt.lu_blocked ? kBatchChain : 0, kTopWidth);
+    {   // a working set beyond LDS runs in MODE 1 / 0, whose kernels take a wider dense top
+        const bool mx = opt.precision != PF_FP64;
+        const long long w0 = mx ? work_bytes<float>(src.nB, b->sym.n, src.nPV, b->sym.nblk())
+                                : work_bytes<double>(src.nB, b->sym.n, src.nPV, b->sym.nblk());
+        if (w0 > kLdsLimit && opt.lu_blocked)
+            b->sym = analyze(src, b->yp, 0, kBatchChainWide, kTopWidth);
+    }
     if (!b->sym.level_ordered()) {
-        g_batch_error = "symbolic analysis: nodes not numbered level by level";
+        pf::set_last_error("symbolic analysis: nodes not numbered level by level");
         return PF_INVALID_ARG;
     }
     const pf_network &net = b->net;
@@ -888,13 +998,18 @@ pf_status pf_batch::occupancy(const BatchNet &nb, int mode, int block, int *per_
 // launch shape of one topology: mode, workgroup size and workgroups per CU
 // (capped by occupancy); grows the global workspace when needed.
 pf_status pf_batch::configure(BatchNet &nb, int mode, int block, int per_cu) {
-    if (mode == 2 && !nb.lds) return PF_INVALID_ARG;
-    if (mode == 1 && (nb.lds || nb.partial == 0)) return PF_INVALID_ARG;
-    if (mode == 0 && nb.lds) return PF_INVALID_ARG;
-    if (block < 64 || block > 1024 || block % 64) return PF_INVALID_ARG;
+    auto refuse = [&](const char *why) {
+        pf::set_last_error(std::string("batch launch shape (mode ") + std::to_string(mode) + ", block " +
+                           std::to_string(block) + "): " + why);
+        return PF_INVALID_ARG;
+    };
+    if (mode == 2 && !nb.lds) return refuse("the working set does not fit in LDS");
+    if (mode == 1 && (nb.lds || nb.partial == 0)) return refuse("mode 1 is for working sets beyond LDS whose step vectors fit");
+    if (mode == 0 && nb.lds) return refuse("the working set fits in LDS: use mode 2");
+    if (block < 64 || block > 1024 || block % 64) return refuse("block must be a multiple of 64 up to 1024");
     int occ = 0;
     BCHK(occupancy(nb, mode, block, &occ));
-    if (occ < 1) return PF_INVALID_ARG;
+    if (occ < 1) return refuse("no workgroup fits on a CU (registers or LDS)");
     per_cu = per_cu <= 0 ? occ : std::min(per_cu, occ);
     nb.mode = mode;
     nb.block = block;
@@ -977,6 +1092,7 @@ pf_status pf_batch_create(const pf_network *const *nets, int n_nets, const pf_op
     hipDeviceProp_t prop;
     BHIP(hipGetDeviceProperties(&prop, device));
     b->cus = prop.multiProcessorCount;
+    BCHK(noalloc_setup(device));
     for (int i = 0; i < n_nets; i++) {
         if (!nets[i] || nets[i]->nB < 2 || nets[i]->slack < 0) return PF_INVALID_ARG;
         BCHK(b->add_net(*nets[i]));
@@ -1001,8 +1117,10 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
     const bool first = b->n_scen == 0;
     if (!first) {
         // a later call refreshes the data of the same scenario layout
-        if (n_scen != b->n_scen || !std::equal(net_of, net_of + n_scen, b->net_of.begin()))
+        if (n_scen != b->n_scen || !std::equal(net_of, net_of + n_scen, b->net_of.begin())) {
+            pf::set_last_error("pf_batch_set_scenarios: a later call must keep the first call's layout");
             return PF_INVALID_ARG;
+        }
     } else {
         b->n_scen = n_scen;
         b->net_of.assign(net_of, net_of + n_scen);
@@ -1013,6 +1131,7 @@ pf_status pf_batch_set_scenarios(pf_batch *b, int n_scen, const int *net_of, con
         for (int s = 0; s < n_scen; s++) {
             if (net_of[s] < 0 || net_of[s] >= (int)b->nets.size()) {
                 b->n_scen = 0;
+                pf::set_last_error("pf_batch_set_scenarios: net_of holds an index outside the batch's networks");
                 return PF_INVALID_ARG;
             }
             b->off[s] = tot;
@@ -1251,3 +1370,109 @@ pf_status pf_batch_get_limq(pf_batch *b, unsigned char *limQ, int *flips) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- allocation-free entry
+
+extern "C" pf_status pf_batch_solve_noalloc(pf_batch *b, int n_scen, const int *net_of, const double *Pliq,
+                                            const double *Qliq, const double *Qload, const double *V0,
+                                            const double *theta0, double *V, double *theta, int *iterations,
+                                            int *converged, double *max_mismatch, unsigned char *limQ,
+                                            int *flips, void *stream) {
+    if (!b || n_scen <= 0 || !net_of || !Pliq || !Qliq || !Qload || !V || !theta || !iterations ||
+        !converged || !max_mismatch || !limQ || !flips || ((V0 == nullptr) != (theta0 == nullptr)))
+        return PF_INVALID_ARG;
+    BHIP(hipSetDevice(b->device));
+    char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch_ptr[b->device];
+    }
+    if (!base) {
+        pf::set_last_error("scratch address not set up on this device");
+        return PF_DEVICE_ERROR;
+    }
+    const int nn = (int)b->nets.size();
+    // layout: header, per-scenario offsets, per-network lists, workspaces
+    auto up = [](long long x) { return (x + 255) & ~255LL; };
+    const long long o_off = kScratchHeader;
+    const long long o_pvoff = up(o_off + 8LL * n_scen);
+    const long long o_scen = up(o_pvoff + 8LL * n_scen);
+    const long long o_ws = up(o_scen + 4LL * n_scen * nn);
+    if (o_ws > kScratchBytes) {
+        pf::set_last_error("pf_batch_solve_noalloc: too many scenarios for the fixed scratch");
+        return PF_INVALID_ARG;
+    }
+    int n_big = 0;
+    for (auto &nb : b->nets) n_big += nb->mode != 2;
+    const long long share = n_big ? (kScratchBytes - o_ws) / n_big : 0;
+    struct Plan { int grid; long long ws_off; };
+    Plan plan[64];            // at most 64 networks per batch (pf_batch_create)
+    long long o = o_ws;
+    for (int i = 0; i < nn; i++) {
+        BatchNet &nb = *b->nets[i];
+        int per_cu = nb.per_cu;
+        if (nb.grid == 0) BCHK(b->occupancy(nb, nb.mode, nb.block, &per_cu));
+        int grid = std::max(1, per_cu * b->cus);
+        if (nb.mode != 2) {
+            grid = (int)std::min<long long>(grid, share / nb.ws);
+            if (grid < 1) {
+                pf::set_last_error("pf_batch_solve_noalloc: a working-set slice does not fit the fixed scratch");
+                return PF_INVALID_ARG;
+            }
+        }
+        plan[i] = {grid, o};
+        if (nb.mode != 2) o += up((long long)grid * nb.ws);
+    }
+    if (o > kScratchBytes) {
+        pf::set_last_error("pf_batch_solve_noalloc: working sets exceed the fixed scratch");
+        return PF_INVALID_ARG;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : b->st[0];
+    int *count = (int *)base, *counter = (int *)(base + 256), *invalid = (int *)(base + 512);
+    long long *off = (long long *)(base + o_off), *pvoff = (long long *)(base + o_pvoff);
+    int *scen = (int *)(base + o_scen);
+    NoallocNets info{};
+    info.n_nets = nn;
+    for (int i = 0; i < nn; i++) {
+        info.nB[i] = b->nets[i]->net.nB;
+        info.nPV[i] = b->nets[i]->net.nPV;
+    }
+    k_noalloc_tables<<<1, 1024, 0, s>>>(net_of, n_scen, info, off, pvoff, scen, count, counter, invalid);
+    BHIP(hipGetLastError());
+    const bool mixed = b->opt.precision != PF_FP64;
+    for (int i = 0; i < nn; i++) {
+        BatchNet &nb = *b->nets[i];
+        BatchJob J{};
+        J.nscen = 0;
+        J.nscen_dev = count + i;
+        J.scen = scen + (long long)i * n_scen;
+        J.off = off;
+        J.Pliq = Pliq; J.Qliq = Qliq; J.Qload = Qload;
+        J.V0 = V0; J.th0 = theta0;
+        J.Vout = V; J.thout = theta;
+        J.iters = iterations; J.conv = converged; J.maxmis = max_mismatch;
+        J.pv_off = pvoff; J.limQout = limQ; J.flips = flips;
+        J.counter = counter + i;
+        J.gws = nb.mode != 2 ? base + plan[i].ws_off : nullptr;
+        J.ws_bytes = nb.ws;
+        J.max_iter = b->opt.no_max_iter;
+        J.lim = b->opt.lim_inj_reat ? 1 : 0;
+        J.refine = mixed ? b->opt.refine_steps : 0;
+        J.tol = b->opt.tolerancia;
+        J.diverge = 1e6;
+        BatchTopo tp = nb.tp;
+        void *args[] = {&tp, &J};
+        BHIP(hipLaunchKernel((const void *)batch_kernel(mixed, nb.mode, nb.block), dim3(plan[i].grid),
+                             dim3(nb.block), args, nb.mode == 0 ? 0 : (size_t)nb.shm, s));
+    }
+    if (!stream) {
+        BHIP(hipStreamSynchronize(s));
+        int bad = 0;
+        BHIP(hipMemcpy(&bad, invalid, sizeof(int), hipMemcpyDeviceToHost));
+        if (bad) {
+            pf::set_last_error("pf_batch_solve_noalloc: net_of holds an index outside the batch's networks");
+            return PF_INVALID_ARG;
+        }
+    }
+    return PF_OK;
+}

@@ -234,7 +234,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_jfill(DevNet N, DevState S, DevSym Y, T *blk, int gate) {
     if (gate && !S.ctl->step) return;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= Y.nslot) return;
+    if (s >= Y.nslot) {   // threads past the slots clear the dense top block S
+        const int q = s - Y.nslot;
+        if (q < Y.ch_m * Y.ch_m) store_b4(blk, Y.n + 3 * Y.nnzL + q, B4<T>{0, 0, 0, 0});
+        return;
+    }
     int e = Y.slot_e[s];
     B4<T> out;
     if (e < 0) {
@@ -415,7 +419,8 @@ __global__ void __launch_bounds__(256) k_chain_schur(DevSym Y, T *blk, Ctl *ctl,
 template <typename T>
 __global__ void __launch_bounds__(ChainTiles<T>::threads) k_chain_invert(DevSym Y, T *blk, Ctl *ctl, int gate) {
     if (gate && !ctl->step) return;
-    chain_invert<T, ChainTiles<T>::tr, ChainTiles<T>::tc, kChainMax>(Y, blk, &ctl->singular);
+    __shared__ B4<T> lds[4 * kChainMax];
+    chain_invert<T, ChainTiles<T>::tr, ChainTiles<T>::tc>(Y, blk, &ctl->singular, lds);
 }
 
 // ----------------------------------------------------- triangular solves
@@ -584,7 +589,8 @@ __global__ void __launch_bounds__(1024) k_fwd_tail(DevSym Y, const T *blk, doubl
         pk = nk;
         pL = nL;
     }
-    if (Y.ch_m > 0) chain_solve<T, 16, kChainMax>(Y, blk, x);
+    __shared__ double s_r[2 * kChainMax];
+    if (Y.ch_m > 0) chain_solve<T, 16>(Y, blk, x, s_r);
 }
 
 template <typename T>

@@ -61,12 +61,13 @@ struct DevState {
 // nodes (pf_internal.hpp), S held in TR x TC register tiles of blocks by one
 // workgroup of 1024 threads
 template <typename T> struct ChainTiles;
-template <> struct ChainTiles<float> { static constexpr int threads = 1024, tr = 4, tc = 3; };
-template <> struct ChainTiles<double> { static constexpr int threads = 1024, tr = 4, tc = 3; };
+template <> struct ChainTiles<float> { static constexpr int threads = 1024, tr = 4, tc = 4, cap = kChainMax; };
+template <> struct ChainTiles<double> { static constexpr int threads = 1024, tr = 4, tc = 3, cap = kChainMaxF64; };
 template <typename T>
 constexpr bool chain_tiles_fit() {
-    return ((kChainMax + ChainTiles<T>::tr - 1) / ChainTiles<T>::tr) *
-               ((kChainMax + ChainTiles<T>::tc - 1) / ChainTiles<T>::tc) <= ChainTiles<T>::threads;
+    constexpr int m = ChainTiles<T>::cap;
+    return m <= kChainMax && ((m + ChainTiles<T>::tr - 1) / ChainTiles<T>::tr) *
+                                     ((m + ChainTiles<T>::tc - 1) / ChainTiles<T>::tc) <= ChainTiles<T>::threads;
 }
 static_assert(chain_tiles_fit<float>() && chain_tiles_fit<double>(), "dense chain does not fit the registers");
 // lanes per destination block in the cooperative narrow-level factorisation

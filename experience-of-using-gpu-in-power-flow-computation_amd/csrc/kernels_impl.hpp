@@ -201,7 +201,8 @@ __device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *
 // so a solve costs two barriers there instead of two levels per node.
 
 // S = chain block minus the updates of the nodes before the chain, written
-// row-major (ch_m x ch_m blocks) at block n + 3 nnzL.  Groups of G lanes
+// row-major (ch_m x ch_m blocks) at block n + 3 nnzL over a zeroed S (the
+// entries outside the pattern of L stay zero).  Groups of G lanes
 // per destination block (group group0 + threadIdx.x / G of ngroups, so that
 // one or many workgroups can share the pass); every destination is
 // independent.  The caller orders it after every node before the chain.
@@ -226,7 +227,7 @@ __device__ void chain_schur(const DevSym &Y, T *blk, int group0, int ngroups) {
                     const B4<T> D = load_b4(blk, j);
                     store_b4(blk, SB + a * m + a, B4<T>{D.a - acc.a, D.b - acc.b, D.c - acc.c, D.d - acc.d});
                 } else {
-                    const int p = Y.colptr[j] + t - 1, i = a + t;   // row node j + t
+                    const int p = Y.colptr[j] + t - 1, i = Y.rowind[p] - c0;
                     const B4<T> Lb = load_b4(blk, L0 + p), Ub = load_b4(blk, U0 + p);
                     store_b4(blk, SB + i * m + a, B4<T>{Lb.a - acc.a, Lb.b - acc.b, Lb.c - acc.c, Lb.d - acc.d});
                     store_b4(blk, SB + a * m + i,
@@ -268,12 +269,11 @@ __device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B
 // during the elimination, as TR x TC tiles of blocks: thread t owns rows
 // (t / ntc) * TR + a and columns (t % ntc) * TC + c, with ntc = ceil(m / TC)
 // (requires ceil(m / TR) * ceil(m / TC) <= blockDim).  Row k and column k go
-// through LDS (double buffered: one barrier per step); a thread reads TR + TC
-// blocks of them per step, not two per block it owns.  The caller orders it
-// after chain_schur and synchronises after.
-template <typename T, int TR, int TC, int MAXM>
-__device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
-    __shared__ B4<T> s_row[2][MAXM], s_col[2][MAXM];
+// through LDS (lds: 4 ch_m blocks, double buffered: one barrier per step); a
+// thread reads TR + TC blocks of them per step, not two per block it owns.
+// The caller orders it after chain_schur and synchronises after.
+template <typename T, int TR, int TC>
+__device__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
     const int m = Y.ch_m, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x;
     const int ntc = (m + TC - 1) / TC, ntr = (m + TR - 1) / TR;
@@ -286,14 +286,14 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
         for (int c = 0; c < TC; c++)
             reg[a][c] = (i0 + a < m && j0 + c < m) ? load_b4(blk, SB + (i0 + a) * m + j0 + c) : B4<T>{0, 0, 0, 0};
     for (int k = 0; k < m; k++) {
-        const int b = k & 1;
+        B4<T> *s_row = lds + (k & 1) * 2 * m, *s_col = s_row + m;
         if (k >= i0 && k < i0 + TR) {   // this thread holds part of row k
 #pragma unroll
             for (int a = 0; a < TR; a++)
                 if (i0 + a == k) {
 #pragma unroll
                     for (int c = 0; c < TC; c++)
-                        if (j0 + c < m) s_row[b][j0 + c] = reg[a][c];
+                        if (j0 + c < m) s_row[j0 + c] = reg[a][c];
                 }
         }
         if (k >= j0 && k < j0 + TC) {   // ... of column k
@@ -302,17 +302,17 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
                 if (j0 + c == k) {
 #pragma unroll
                     for (int a = 0; a < TR; a++)
-                        if (i0 + a < m) s_col[b][i0 + a] = reg[a][c];
+                        if (i0 + a < m) s_col[i0 + a] = reg[a][c];
                 }
         }
         __syncthreads();
-        const B4<T> P = pivot_inverse(s_row[b][k], bad);
+        const B4<T> P = pivot_inverse(s_row[k], bad);
         if constexpr (sizeof(T) == 4) {   // registers to spare: read row and column once
             B4<T> pr[TC], ci[TR];
 #pragma unroll
-            for (int c = 0; c < TC; c++) pr[c] = (j0 + c < m && j0 + c != k) ? mul(P, s_row[b][j0 + c]) : P;
+            for (int c = 0; c < TC; c++) pr[c] = (j0 + c < m && j0 + c != k) ? mul(P, s_row[j0 + c]) : P;
 #pragma unroll
-            for (int a = 0; a < TR; a++) ci[a] = s_col[b][i0 + a < m ? i0 + a : 0];
+            for (int a = 0; a < TR; a++) ci[a] = s_col[i0 + a < m ? i0 + a : 0];
 #pragma unroll
             for (int a = 0; a < TR; a++) {
                 const int i = i0 + a;
@@ -337,7 +337,7 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
         for (int c = 0; c < TC; c++) {
             const int j = j0 + c;
             if (j >= m) continue;
-            const B4<T> pr = (j != k) ? mul(P, s_row[b][j]) : P;
+            const B4<T> pr = (j != k) ? mul(P, s_row[j]) : P;
 #pragma unroll
             for (int a = 0; a < TR; a++) {
                 const int i = i0 + a;
@@ -345,10 +345,10 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
                 if (i == k) {
                     reg[a][c] = pr;
                 } else if (j == k) {
-                    const B4<T> v = mul(s_col[b][i], P);
+                    const B4<T> v = mul(s_col[i], P);
                     reg[a][c] = B4<T>{-v.a, -v.b, -v.c, -v.d};
                 } else {
-                    mms(reg[a][c], s_col[b][i], pr);
+                    mms(reg[a][c], s_col[i], pr);
                 }
             }
         }
@@ -360,21 +360,25 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad) {
             if (i0 + a < m && j0 + c < m) store_b4(blk, SB + (i0 + a) * m + j0 + c, reg[a][c]);
 }
 
-// Schur complement and inverse in one workgroup with no registers to spare:
-// S in registers, one block per thread, when the workgroup has ch_m^2
-// threads or more; otherwise S is updated where it is stored (LDS or a
-// global slice), two barriers per step.  Both do the same arithmetic.
-template <typename T, int G, int UNR, int MAXM>
-__device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad) {
-    __shared__ B4<T> s_row[MAXM], s_col[MAXM];
+// Schur complement and inverse of the dense top block in one workgroup (lds:
+// 4 ch_m blocks).  S sits in registers in the smallest square tiles of at
+// most TMAX x TMAX blocks that the workgroup covers; when none does, S is
+// updated where it is stored (LDS or a global slice), two barriers per
+// step.  Every variant does the same arithmetic.
+template <typename T, int G, int UNR, int TMAX>
+__device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
     const int m = Y.ch_m, mm = m * m, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    for (int q = tid; q < mm; q += bd) store_b4(blk, SB + q, B4<T>{0, 0, 0, 0});   // outside L: zero
+    __syncthreads();
     chain_schur<T, G, UNR>(Y, blk, 0, bd / G);
     __syncthreads();
-    if (mm <= bd) {   // one block per thread fits: keep S in registers
-        chain_invert<T, 1, 1, MAXM>(Y, blk, bad);
-        return;
-    }
+    auto fits = [&](int t) { return ((m + t - 1) / t) * ((m + t - 1) / t) <= bd; };
+    if (fits(1)) { chain_invert<T, 1, 1>(Y, blk, bad, lds); return; }
+    if (TMAX >= 2 && fits(2)) { chain_invert<T, 2, 2>(Y, blk, bad, lds); return; }
+    if (TMAX >= 3 && fits(3)) { chain_invert<T, (TMAX >= 3 ? 3 : 1), (TMAX >= 3 ? 3 : 1)>(Y, blk, bad, lds); return; }
+    if (TMAX >= 4 && fits(4)) { chain_invert<T, (TMAX >= 4 ? 4 : 1), (TMAX >= 4 ? 4 : 1)>(Y, blk, bad, lds); return; }
+    B4<T> *s_row = lds, *s_col = lds + m;
     for (int k = 0; k < m; k++) {
         for (int t = tid; t < m; t += bd) {
             s_row[t] = load_b4(blk, SB + k * m + t);
@@ -394,10 +398,10 @@ __device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad) {
 
 // Chain part of a solve, after the forward sweep over the levels before the
 // chain: x_C <- S^-1 (x_C - L_CN x_N), where x_C still holds the right-hand
-// side.  The backward sweep then starts below the chain.  Two barriers.
-template <typename T, int G, int MAXM>
-__device__ void chain_solve(const DevSym &Y, const T *blk, double *x) {
-    __shared__ double s_r[2 * MAXM];
+// side (s_r: 2 ch_m doubles of LDS).  The backward sweep then starts below
+// the chain.  Two barriers.
+template <typename T, int G>
+__device__ void chain_solve(const DevSym &Y, const T *blk, double *x, double *s_r) {
     const int m = Y.ch_m, c0 = Y.ch_c0;
     const int LR0 = Y.n + 2 * Y.nnzL, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x, grp = tid / G, lane = tid % G, ng = (int)blockDim.x / G;

@@ -202,12 +202,11 @@ __global__ void k_denorm(const float *Y, int B, int n, const float *mean, const 
 using namespace pf;
 
 namespace {
-thread_local std::string g_mlp_error;
 #define MHIP(expr)                                                                 \
     do {                                                                           \
         hipError_t e_ = (expr);                                                    \
         if (e_ != hipSuccess) {                                                    \
-            g_mlp_error = std::string(#expr) + ": " + hipGetErrorString(e_);       \
+            pf::set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
             return PF_DEVICE_ERROR;                                                \
         }                                                                          \
     } while (0)

@@ -23,8 +23,13 @@ struct pf_network {
 
 namespace pf {
 
-// longest trailing dense chain the single-network solver factors as one block
-constexpr int kChainMax = 101;
+// most nodes of the dense top block the single-network solver factors as
+// one block: FP32 factor (kChainMax) and FP64 factor (kChainMaxF64), as many
+// as its register tiles hold
+constexpr int kChainMax = 128;
+constexpr int kChainMaxF64 = 101;
+// widest elimination-tree level the dense top block may take
+constexpr int kTopWidth = 1;
 // elimination-tree levels narrower than this run inside one workgroup (the
 // single-network solver's tail); the levels below it form subtree groups
 constexpr int kTailWidth = 64;
@@ -70,9 +75,9 @@ struct Symbolic {
     // the factor also stores L row by row for the forward solve: L position
     // p is copied to block n + 2 nnzL + lr_pos[p] (lr_pos inverts rs_p)
     std::vector<int> lr_pos;
-    // trailing dense chain: the last ch_m nodes [ch_c0, n), one per level
-    // (levels [nlev_s, nlev)), each column holding every later chain node.
-    // Their Schur complement S (ch_m x ch_m blocks) is formed with the
+    // dense top ("chain"): the last ch_m nodes [ch_c0, n), the levels
+    // [nlev_s, nlev) of the elimination tree.  Their Schur complement S
+    // (ch_m x ch_m blocks, zero outside the pattern of L) is formed with the
     // updates of the nodes before ch_c0 and inverted densely; S^-1 is stored
     // row-major at block n + 3 nnzL.  ch_split[e]: for chain destination
     // fd_ptr[ch_c0] + e, the end of its update pairs from nodes < ch_c0.
@@ -111,8 +116,9 @@ struct Symbolic {
     }
 };
 
-// chain_cap: longest trailing dense chain factored as one block (0: none)
-Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap);
+// chain_cap: most nodes of the dense top factored as one block (0: none);
+// top_width: widest elimination-tree level it may take
+Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap, int top_width);
 
 constexpr int kChunkPairs = 16;
 constexpr int kChunkDiag = 1 << 30, kChunkMask = kChunkDiag - 1;
@@ -133,6 +139,9 @@ std::vector<int> min_degree(int n, const std::vector<int> &rowptr, const std::ve
 // zero/non-finite pivot.
 bool host_block_factor(const Symbolic &s, double *blocks);
 void host_block_solve(const Symbolic &s, const double *blocks, double *x /* 2n, node order */);
+
+// text of the last failure on this thread, read back by pf_last_error()
+void set_last_error(const std::string &what);
 
 // data.txt parsing
 void options_default(pf_options &o);

@@ -47,6 +47,8 @@ thread_local std::string g_last_error;
 
 }  // namespace
 
+void pf::set_last_error(const std::string &what) { g_last_error = what; }
+
 struct pf_solver {
     pf_network net;
     pf_options opt;
@@ -71,6 +73,20 @@ struct pf_solver {
     std::vector<double> hblk, hx;
     FlowBranch *flow_br = nullptr;   // branch-flow constants (built on first use)
     double *flow_out = nullptr;
+    // stage timing (the reference's timers, CPU/benchmarks.h:41-52): HIP
+    // events around the timed phases of every pass, eager solves only
+    bool timing = false;
+    hipEvent_t tev[6] = {};
+    pf_timing times{};
+    double hybrid_linear_ms = 0;   // host factor + solve of the last hibridoC pass
+    // per-pass trace for the caller (max |g|, sum g, PV<->PQ flips)
+    pf_pass_callback trace_cb = nullptr;
+    void *trace_user = nullptr;
+    std::vector<unsigned char> trace_lq;
+    std::vector<double> trace_ql, trace_g;
+    void mark(int i) {
+        if (timing) (void)hipEventRecord(tev[i], stream);
+    }
 
     template <typename T>
     pf_status alloc(T **p, size_t n) {
@@ -103,6 +119,8 @@ struct pf_solver {
         if (graph) (void)hipGraphDestroy(graph);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto &e : tev)
+            if (e) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
         if (ctl_host) (void)hipHostFree(ctl_host);
         if (stream) (void)hipStreamDestroy(stream);
@@ -124,6 +142,8 @@ struct pf_solver {
     pf_status solve(pf_result *r);
     pf_status run_hybrid_step();
     pf_status ensure_dense();
+    pf_status account_pass(bool step, double *pass_ms);
+    pf_status trace_pass(int pass, const Ctl &c, const double *pass_ms);
 };
 
 // ------------------------------------------------------------------ build
@@ -138,7 +158,9 @@ pf_status pf_solver::build() {
     // lu_blocked = 0, the straightforward level schedule: one launch per
     // elimination-tree level, levels narrower than kPlainTailWidth nodes in
     // one workgroup, no dense chain and no subtree groups
-    sym = analyze(net, yp, opt.lu_blocked ? kTailWidth : kPlainTailWidth, opt.lu_blocked ? kChainMax : 0);
+    const int chain_cap = opt.precision == PF_MIXED ? ChainTiles<float>::cap : ChainTiles<double>::cap;
+    sym = analyze(net, yp, opt.lu_blocked ? kTailWidth : kPlainTailWidth, opt.lu_blocked ? chain_cap : 0,
+                  kTopWidth);
     if (!sym.level_ordered() || (sym.ch_m > 0 && sym.tail_start > sym.nlev_s)) {
         g_last_error = "symbolic analysis: nodes not numbered level by level, or chain outside the tail";
         return PF_INVALID_ARG;
@@ -256,9 +278,19 @@ pf_status pf_solver::build() {
     PFCHK(alloc(&blk64, 4 * (size_t)sym.nblk()));
     if (opt.precision == PF_MIXED) PFCHK(alloc(&blk32, 4 * (size_t)sym.nblk()));
 
-    // Ybus on the device (once per network)
+    // Ybus on the device (once per network), timed like the reference's
+    // admitancia phase (CPU/main.cpp:47-56)
+    for (auto &e : tev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(tev[0], stream));
     k_ybus<<<(yp.nnz + 255) / 256, 256, 0, stream>>>(dn, yp.nnz);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(tev[1], stream));
+    HIPCHK(hipEventSynchronize(tev[1]));
+    {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, tev[0], tev[1]));
+        times.ybus_ms = ms;
+    }
     PFCHK(reset_state());
     HIPCHK(hipStreamSynchronize(stream));
     return PF_OK;
@@ -307,7 +339,9 @@ pf_status pf_solver::reset_ctl() {
 
 pf_status pf_solver::enqueue_head(int gate) {
     const int nB = net.nB;
+    mark(0);
     k_pq<4><<<(nB + 63) / 64, 256, 0, stream>>>(dn, ds, gate);
+    mark(1);
     if (lim()) k_limits<<<1, 1024, 0, stream>>>(dn, ds, 0, 1);
     k_mismatch<<<(nB + 255) / 256, 256, 0, stream>>>(dn, ds, gate);
     k_control<<<1, 64, 0, stream>>>(ds, lim());
@@ -318,7 +352,11 @@ pf_status pf_solver::enqueue_head(int gate) {
 
 template <typename T>
 pf_status pf_solver::enqueue_lu(T *blk, int gate) {
-    k_jfill<T><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk, gate);
+    // the threads past the slots clear the dense top block
+    const int nfill = sym.nslot() + sym.ch_m * sym.ch_m;
+    mark(2);
+    k_jfill<T><<<(nfill + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk, gate);
+    mark(3);
     if (sym.n_parts > 0) {
         k_factor_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, ctl, gate);
     } else {
@@ -369,7 +407,9 @@ pf_status pf_solver::enqueue_factor_solve(int gate) {
     const int m = 2 * sym.n;
     if (opt.metodo == PF_DENSO) {
         PFCHK(ensure_dense());
+        mark(2);
         k_jfill<double><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk64, gate);
+        mark(3);
         size_t mm = (size_t)m * m;
         k_fill<<<(unsigned)((mm + 255) / 256), 256, 0, stream>>>(dense, mm, 0.0);
         k_dense_scatter<<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dy, blk64, dense, m, ctl, gate);
@@ -395,7 +435,9 @@ pf_status pf_solver::enqueue_factor_solve(int gate) {
 
 pf_status pf_solver::enqueue_step(int gate) {
     PFCHK(enqueue_factor_solve(gate));
+    mark(4);
     k_update<<<(net.nB + 255) / 256, 256, 0, stream>>>(dn, ds, gate);
+    mark(5);
     HIPCHK(hipGetLastError());
     return PF_OK;
 }
@@ -404,8 +446,11 @@ pf_status pf_solver::enqueue_step(int gate) {
 // same symbolic program (the reference: PARDISO, GPU/solverCPU.cpp:218-330).
 pf_status pf_solver::run_hybrid_step() {
     const size_t ns = sym.nslot();
+    mark(2);
     k_jfill<double><<<(ns + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk64, 1);
     HIPCHK(hipGetLastError());
+    mark(3);
+    auto h0 = std::chrono::steady_clock::now();
     hblk.assign(4 * (size_t)sym.nblk(), 0.0);   // slots, then room for the dense chain
     hx.resize(2 * (size_t)sym.n);
     PFCHK(get(hblk.data(), blk64, 4 * ns));
@@ -418,20 +463,81 @@ pf_status pf_solver::run_hybrid_step() {
         host_block_solve(sym, hblk.data(), hx.data());
     }
     PFCHK(put(ds.x, hx.data(), hx.size()));
+    HIPCHK(hipStreamSynchronize(stream));
+    hybrid_linear_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    mark(4);
     k_update<<<(net.nB + 255) / 256, 256, 0, stream>>>(dn, ds, 1);
     HIPCHK(hipGetLastError());
+    mark(5);
+    return PF_OK;
+}
+
+// stage times of one eager pass whose events have completed (P/Q,
+// Jacobian, linear system, update), added to the totals
+pf_status pf_solver::account_pass(bool step, double *pass_ms) {
+    for (int i = 0; i < 4; i++) pass_ms[i] = 0;
+    if (!timing) return PF_OK;
+    auto el = [&](int a, int b) {
+        float ms = 0;
+        return hipEventElapsedTime(&ms, tev[a], tev[b]) == hipSuccess ? (double)ms : 0.0;
+    };
+    pass_ms[0] = el(0, 1);
+    if (step) {
+        pass_ms[1] = el(2, 3);
+        pass_ms[2] = opt.metodo == PF_HIBRIDO_C ? hybrid_linear_ms : el(3, 4);
+        pass_ms[3] = el(4, 5);
+    }
+    times.pq_ms += pass_ms[0];
+    times.jacobian_ms += pass_ms[1];
+    times.linear_ms += pass_ms[2];
+    times.update_ms += pass_ms[3];
+    times.passes++;
+    return PF_OK;
+}
+
+// hands one pass to the trace callback: max |g|, the sum of g and the
+// PV<->PQ flips of the pass (compared with the state before it), with the
+// limit each flip concerns (0 PV->PQ at Qmin, 1 PV->PQ at Qmax, 2 back to
+// PV from Qmax, 3 back to PV from Qmin: CPU/limInjReat.cpp:38-91)
+pf_status pf_solver::trace_pass(int pass, const Ctl &c, const double *pass_ms) {
+    if (!trace_cb) return PF_OK;
+    const int nPV = net.nPV;
+    std::vector<unsigned char> lq(nPV);
+    std::vector<double> ql(net.nB), g(2 * (size_t)sym.n);
+    if (nPV) PFCHK(get(lq.data(), ds.limQ, nPV));
+    PFCHK(get(ql.data(), ds.QliqLim, ql.size()));
+    PFCHK(get(g.data(), ds.g, g.size()));
+    double sum = 0;
+    for (double v : g) sum += v;
+    std::vector<int> idx, kind;
+    for (int i = 0; i < nPV; i++) {
+        if (lq[i] == trace_lq[i]) continue;
+        const int b = net.pv[i];
+        const bool at_max = (lq[i] ? ql[b] : trace_ql[b]) == net.Qmax[i] - Qload[b];
+        idx.push_back(i);
+        kind.push_back(lq[i] ? (at_max ? 1 : 0) : (at_max ? 2 : 3));
+    }
+    trace_cb(trace_user, pass, c.maxG, sum, c.post, (int)idx.size(), idx.data(), kind.data(), pass_ms);
+    trace_lq = lq;
+    trace_ql = ql;
     return PF_OK;
 }
 
 pf_status pf_solver::solve(pf_result *res) {
     HIPCHK(hipSetDevice(device));
     PFCHK(reset_ctl());
+    if (timing) {
+        const double yb = times.ybus_ms, fl = times.flows_ms;
+        times = pf_timing{};
+        times.ybus_ms = yb;
+        times.flows_ms = fl;
+    }
     auto t0 = std::chrono::steady_clock::now();
     k_init_E<<<(net.nB + 255) / 256, 256, 0, stream>>>(dn, ds);
     HIPCHK(hipGetLastError());
     const int maxit = opt.no_max_iter;
     Ctl c{};
-    if (opt.use_graph && opt.metodo == PF_ESPARSO) {
+    if (opt.use_graph && opt.metodo == PF_ESPARSO && !timing && !trace_cb) {
         if (!gexec) {
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             pf_status st = enqueue_head(1);
@@ -462,22 +568,34 @@ pf_status pf_solver::solve(pf_result *res) {
         c = ctl_host[(checked - 1) & 1];
         PFCHK(get(&c, ctl, 1));
     } else {
+        if (trace_cb) {
+            trace_lq.assign(net.nPV, 0);
+            trace_ql = Qliq;
+            if (net.nPV) PFCHK(get(trace_lq.data(), ds.limQ, net.nPV));
+            PFCHK(get(trace_ql.data(), ds.QliqLim, trace_ql.size()));
+        }
+        double pass_ms[4];
         for (int it = 0; it < maxit; it++) {
             PFCHK(enqueue_head(1));
             if (opt.metodo == PF_HIBRIDO_C) {
                 PFCHK(get(&c, ctl, 1));
-                if (c.done) break;
+                const Ctl head = c;
                 if (c.step) PFCHK(run_hybrid_step());
                 PFCHK(get(&c, ctl, 1));
-                if (c.singular) break;
+                PFCHK(account_pass(head.step != 0, pass_ms));
+                PFCHK(trace_pass(it, head, pass_ms));
+                if (c.done || c.singular) break;
                 continue;
             }
             PFCHK(enqueue_step(1));
             PFCHK(get(&c, ctl, 1));
+            PFCHK(account_pass(c.step != 0, pass_ms));
+            PFCHK(trace_pass(it, c, pass_ms));
             if (c.done || c.singular) break;
         }
     }
     auto t1 = std::chrono::steady_clock::now();
+    if (timing) times.iterative_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     res->iterations = c.it;
     res->converged = c.done && !c.singular;
     res->singular = c.singular;
@@ -575,6 +693,25 @@ pf_status pf_solver_lu_info(pf_solver *s, int *n, int *nnzL, int *levels, int *t
     return PF_OK;
 }
 
+pf_status pf_solver_set_timing(pf_solver *s, int on) {
+    if (!s) return PF_INVALID_ARG;
+    s->timing = on != 0;
+    return PF_OK;
+}
+
+pf_status pf_solver_get_timing(pf_solver *s, pf_timing *t) {
+    if (!s || !t) return PF_INVALID_ARG;
+    *t = s->times;
+    return PF_OK;
+}
+
+pf_status pf_solver_set_trace(pf_solver *s, pf_pass_callback cb, void *user) {
+    if (!s) return PF_INVALID_ARG;
+    s->trace_cb = cb;
+    s->trace_user = user;
+    return PF_OK;
+}
+
 const char *pf_last_error(void) { return g_last_error.c_str(); }
 
 // ------------------------------------------------------------ stages
@@ -651,7 +788,10 @@ pf_status pf_solver::flows(double *Pdp, double *Qdp, double *Ppd, double *Qpd) {
 pf_status pf_solver_flows(pf_solver *s, double *Pdp, double *Qdp, double *Ppd, double *Qpd) {
     if (!s) return PF_INVALID_ARG;
     HIPCHK(hipSetDevice(s->device));
-    return s->flows(Pdp, Qdp, Ppd, Qpd);
+    auto t0 = std::chrono::steady_clock::now();   // the reference's fluxo phase (CPU/main.cpp:103-118)
+    pf_status st = s->flows(Pdp, Qdp, Ppd, Qpd);
+    if (s->timing) s->times.flows_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return st;
 }
 
 pf_status pf_stage_flows(pf_solver *s, const double *V, const double *theta, double *Pdp, double *Qdp,

@@ -280,7 +280,7 @@ Symbolic analyze_order(const pf_network &net, const YPattern &yp, int tail_width
 // contiguous range of nodes (lvl_nodes is the identity) and the factor
 // program, the row structures and the flattened destinations of a level are
 // contiguous too.
-Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap) {
+Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int chain_cap, int top_width) {
     const int nB = net.nB, sl = net.slack;
     const int n = nB - 1;
     auto blk = [&](int k) { return k - (k > sl ? 1 : 0); };
@@ -301,16 +301,21 @@ Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int 
     std::vector<int> perm(n);
     for (int t = 0; t < n; t++) perm[t] = md[by_level[t]];
     Symbolic s = analyze_order(net, yp, tail_width, perm, gp, gi);
-    // trailing dense chain: node n-1-i has exactly i entries below its pivot
-    // (then they are all the later nodes); keep it level-aligned (its first
-    // node alone on its level) and at most chain_cap long
-    int m = 0;
-    while (m < n && m < chain_cap && s.colptr[n - m] - s.colptr[n - 1 - m] == m) m++;
-    if (m > 0 && s.lvl_ptr[s.level[n - m]] != n - m) m--;
-    if (m < 4) m = 0;
+    // dense top: the last elimination-tree levels, each at most top_width
+    // nodes wide, at most chain_cap nodes in all.  They are factored as one
+    // dense block (its Schur complement, inverted), so the level walks end
+    // below them; entries of the block outside the pattern of L are zero.
+    int m = 0, taken = 0;
+    while (taken < s.nlev) {
+        const int l = s.nlev - 1 - taken, w = s.lvl_ptr[l + 1] - s.lvl_ptr[l];
+        if (w > top_width || m + w > chain_cap) break;
+        m += w;
+        taken++;
+    }
+    if (m < 4) m = taken = 0;
     s.ch_m = m;
     s.ch_c0 = n - m;
-    s.nlev_s = s.nlev - m;
+    s.nlev_s = s.nlev - taken;
     if (m > 0) {
         std::vector<int> pcol(s.nnzL);
         for (int k = 0; k < n; k++)
@@ -413,6 +418,7 @@ inline void mul2(double *c, const double *a, const double *b) {   // c = a*b (2x
 bool host_chain_factor(const Symbolic &s, double *blk) {
     const int n = s.n, m = s.ch_m, c0 = s.ch_c0, L0 = n, U0 = n + s.nnzL, SB = n + 3 * s.nnzL;
     double *S = blk + 4 * (size_t)SB;
+    std::fill(S, S + 4 * (size_t)m * m, 0.0);   // entries outside the pattern of L
     auto at = [&](int i, int j) { return S + 4 * ((size_t)i * m + j); };
     const int dbase = s.fd_ptr[c0];
     for (int j = c0; j < n; j++)
@@ -425,7 +431,7 @@ bool host_chain_factor(const Symbolic &s, double *blk) {
                     mms(D, blk + 4 * (size_t)(L0 + s.fc_a[q]), blk + 4 * (size_t)(U0 + s.fc_b[q]));
                 std::copy(D, D + 4, at(a, a));
             } else {
-                const int p = s.colptr[j] + t - 1, i = a + t;
+                const int p = s.colptr[j] + t - 1, i = s.rowind[p] - c0;
                 double Lb[4], Ub[4];
                 std::copy(blk + 4 * (size_t)(L0 + p), blk + 4 * (size_t)(L0 + p) + 4, Lb);
                 std::copy(blk + 4 * (size_t)(U0 + p), blk + 4 * (size_t)(U0 + p) + 4, Ub);

@@ -150,6 +150,32 @@ pf_status pf_solver_get_state(pf_solver *s, double *V, double *theta,
  * 2x2 blocks, elimination-tree levels, levels run by one workgroup */
 pf_status pf_solver_lu_info(pf_solver *s, int *n, int *nnzL, int *levels, int *tail_levels);
 
+/* ------------------------------------------------ timing and iteration trace */
+/* The reference's phase timers (timerType, CPU/benchmarks.h:41-89; phases
+ * timed in CPU/main.cpp:41-118 and CPU/newtonRaphson.cpp:179-305), in ms:
+ * Ybus assembly at pf_solver_create, then per pf_solve summed over the
+ * passes (P/Q, Jacobian, linear system with its refinement, state update),
+ * the whole NR loop, and the last pf_solver_flows.  GPU phases are timed
+ * with HIP events, so timing runs the NR loop with one host sync per pass
+ * (use_graph is ignored while it is on).                                    */
+typedef struct pf_timing {
+    double ybus_ms, pq_ms, jacobian_ms, linear_ms, update_ms, iterative_ms, flows_ms;
+    int passes;             /* NR passes timed (updates applied + the last check)  */
+} pf_timing;
+pf_status pf_solver_set_timing(pf_solver *s, int on);
+pf_status pf_solver_get_timing(pf_solver *s, pf_timing *t);
+/* Called after every NR pass of pf_solve (one host sync per pass while set):
+ * the pass, max|g| and the sum of g at its mismatch evaluation
+ * (CPU/newtonRaphson.cpp:248-251), whether it is a switch taken at
+ * convergence (CPU/newtonRaphson.cpp:253-257), and its PV<->PQ flips: PV
+ * index and kind (0 PV->PQ at Qmin, 1 PV->PQ at Qmax, 2 PQ at Qmax -> PV,
+ * 3 PQ at Qmin -> PV; the messages of CPU/limInjReat.cpp:46-80), and the
+ * pass's stage times in ms (P/Q, Jacobian, linear system, update; zeros
+ * when timing is off).  NULL: off.                                          */
+typedef void (*pf_pass_callback)(void *user, int pass, double max_g, double sum_g, int at_convergence,
+                                 int n_flips, const int *pv_index, const int *kind, const double *stage_ms);
+pf_status pf_solver_set_trace(pf_solver *s, pf_pass_callback cb, void *user);
+
 /* ---------------------------------------------------- stage entry points */
 /* Host buffers in and out; each stage runs its own kernels on the GPU.      */
 /* Ybus CSR (0-based, sorted columns), y interleaved re/im: 2*nnz doubles.   */

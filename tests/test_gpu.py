@@ -306,3 +306,33 @@ def test_nr_not_converged_and_reset(pf, cases):
     r2 = s.solve()
     assert r2["converged"] and r2["iterations"] == r["iterations"]
     assert np.array_equal(s.state()[0], V1)
+
+
+@pytest.mark.parametrize("variant", ["eager", "graph", "mixed", "denso", "hibridoC", "batch"])
+def test_nr_traced_limits(pf, oracle, variant):
+    """The 3-bus case whose switch at iteration 1 and back-switch at
+    iteration 2 are traced by hand in test_oracle.py: every solver path takes
+    the same decisions at the same passes and keeps the back-switched |V|."""
+    p = os.path.join(GOLDEN, "c3_limits.cdf")
+    net = pf.Network(p)
+    o = oracle.Case(p).solve(tol=1e-8, lim=True, reuse_symbolic=True)
+    _say("traced limits", variant)
+    if variant == "batch":
+        b = pf.Batch([net], pf.options(tolerancia=1e-8, no_max_iter=100, lim_inj_reat=1))
+        b.set_scenarios(np.zeros(1, np.int32), net.Pliq, net.Qliq, net.Qload)
+        b.solve()
+        V, th, it, cv, mm = b.results()
+        lq, fl = b.limq()
+        r = {"iterations": int(it[0]), "converged": bool(cv[0]), "n_switch_events": int(fl[0])}
+        lq = lq[0]
+    else:
+        kw = {"eager": {}, "graph": {"use_graph": 1}, "mixed": {"use_graph": 1, "precision": pf.MIXED},
+              "denso": {"metodo": pf.DENSO}, "hibridoC": {"metodo": pf.HIBRIDO_C}}[variant]
+        s = _solver(pf, net, lim_inj_reat=1, **kw)
+        r = s.solve()
+        V, th, lq = s.state()
+    assert r["converged"] and o["converged"]
+    assert r["iterations"] == o["iterations"] == 4
+    assert r["n_switch_events"] == o["n_switch_events"] == 2
+    assert np.array_equal(lq, o["limQ"])
+    assert np.abs(V - o["V"]).max() <= 1e-9 and np.abs(th - o["theta"]).max() <= 1e-9

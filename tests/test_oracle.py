@@ -97,6 +97,28 @@ def _mutate(path, fn, tmp_path, name):
     return _write(tmp_path, name, "\n".join(fn(lines)))
 
 
+@pytest.mark.parametrize("name", ["ieee14", "ieee118"])
+def test_reader_cdfx_recolumned(oracle, name):
+    """The CDFX column map (CPU/leitura/cdf.cpp:329-493) pinned against the
+    CDF one: tests/golden/make_cdfx.py re-columns the IEEE file (one more
+    column for the bus number, two more for the branch's to-bus), and the
+    oracle reads the same network from both layouts."""
+    sys.path.insert(0, GOLDEN)
+    import make_cdfx
+    src = os.path.join(IEEE, name + ".cdf")
+    with open(src, newline="") as f:
+        text = f.read()
+    with open(os.path.join(IEEE, name + ".cdfx"), newline="") as f:
+        assert f.read() == make_cdfx.recolumn(text)
+    a = oracle.Case(src)
+    b = oracle.Case(os.path.join(IEEE, name + ".cdfx"))
+    for attr in ("nB", "nL", "nPV", "nPQ", "slack", "n_duplicates"):
+        assert getattr(a, attr) == getattr(b, attr), attr
+    for attr in ("V", "theta", "Pliq", "Qliq", "Pload", "Qload", "Pg", "Qg", "Vbase", "gsh", "bsh", "pv", "pq",
+                 "Qmin", "Qmax", "Vset", "frm", "to", "r", "x", "bch", "tap_re", "tap_im", "phi"):
+        assert np.array_equal(getattr(a, attr), getattr(b, attr)), attr
+
+
 def test_reader_errors(oracle, tmp_path):
     src = case_path("c14")
     two_slack = _mutate(src, lambda L: L[:3] + [L[3][:24] + " 3" + L[3][26:]] + L[4:], tmp_path, "a.cdf")
@@ -292,6 +314,93 @@ def test_nr_batch_matches_single(oracle):
         r = c.solve(Pliq=Pl[s], Qliq=Ql[s], Qload=Qld[s])
         assert cv[s] == r["converged"] and it[s] == r["iterations"]
         assert np.array_equal(V[s], r["V"])
+
+
+# ------------------------------------- NR with limits: a traced known answer
+
+def _nr_prescribed(Y, V, th, Psp, Qsp, is_q, schedule, max_iter=20, tol=1e-8):
+    """Dense polar NR in numpy (finite-difference Jacobian) whose PV/PQ changes
+    are PRESCRIBED: schedule[pass] = [(bus, has_q_equation, Q spec)], applied
+    before that pass's mismatch.  No limit rule is evaluated here; a bus that
+    loses its Q equation keeps the |V| it has.  Returns the final state and
+    per-pass records (max |g| at the pass, max |dx| of its step, Q and V)."""
+    V, th, is_q, Qsp = V.copy(), th.copy(), is_q.copy(), Qsp.copy()
+    n, rec = len(V), []
+    for it in range(max_iter):
+        for b, q, qs in schedule.get(it, []):
+            is_q[b], Qsp[b] = q, qs
+        rp = list(range(1, n))
+        rq = [k for k in rp if is_q[k]]
+
+        def calc(t_, v_):
+            E = v_ * np.exp(1j * t_)
+            S = E * np.conj(Y @ E)
+            return np.concatenate([S.real[rp], S.imag[rq]]), S
+
+        F, S = calc(th, V)
+        g = np.concatenate([Psp[rp], Qsp[rq]]) - F
+        rec.append({"maxg": float(np.abs(g).max()), "Q": S.imag.copy(), "V": V.copy()})
+        if np.abs(g).max() <= tol:
+            return V, th, it, rec
+        x0 = np.concatenate([th[rp], V[rq]])
+        J = np.zeros((len(x0), len(x0)))
+        for j in range(len(x0)):
+            e = np.zeros(len(x0))
+            e[j] = 1e-7
+
+            def at(x):
+                t_, v_ = th.copy(), V.copy()
+                t_[rp], v_[rq] = x[:len(rp)], x[len(rp):]
+                return calc(t_, v_)[0]
+            J[:, j] = (at(x0 + e) - at(x0 - e)) / 2e-7
+        dx = np.linalg.solve(J, g)
+        th[rp] += dx[:len(rp)]
+        V[rq] += dx[len(rp):]
+        rec[-1]["dx"] = float(np.abs(dx).max())
+    return V, th, max_iter, rec
+
+
+def test_nr_limits_traced_switch_and_back_switch(oracle):
+    """tests/golden/c3_limits.cdf: slack bus 1, PV bus 2 (|V| set 0.97,
+    64 MW, Q limits [-1, 100] MVAr), PQ bus 3 (160 MW + 60 MVAr load).
+    Traced by hand from the plain NR passes (numpy, below):
+      pass 0: no check (the reference checks from iteration 1 on);
+      pass 1: the previous step is max|dx| = 0.086 < 0.1, so chkLimQ runs
+              (CPU/newtonRaphson.cpp:202-212) although max|g| = 0.125 >= 0.1
+              at this pass; Q_2 = -0.0181 < Qmin - folga = -0.0101 -> bus 2
+              becomes PQ at Qmin (CPU/limInjReat.cpp:44-51);
+      pass 2: bus 2 now absorbs 0.01 pu only, so its |V| fell to 0.96447
+              < Vset - folga; it is limited at Qmin, so it goes back to PV
+              (CPU/limInjReat.cpp:73-82) KEEPING |V| = 0.96447;
+      then:   Q_2 stays within its limits; converged after 4 updates.
+    A trigger on the current mismatch instead of the previous step would
+    skip pass 1 and never switch; an undo/back-switch with the wrong
+    voltage test would leave bus 2 limited.  Both fail here."""
+    c = oracle.Case(os.path.join(GOLDEN, "c3_limits.cdf"))
+    assert (c.nB, c.nPV, c.nPQ) == (3, 1, 1)
+    Y = ybus_dense(c)
+    is_q = np.zeros(3, bool)
+    is_q[c.pq] = True
+    folga = 1e-4
+    # the plain NR passes fix the trace (no limit logic involved)
+    _, _, _, plain = _nr_prescribed(Y, c.V, c.theta, c.Pliq, c.Qliq, is_q, {})
+    assert plain[0]["dx"] < 0.1 <= plain[1]["maxg"]
+    assert plain[1]["Q"][1] + c.Qload[1] < c.Qmin[0] - folga
+    Vp, thp, itp, rec = _nr_prescribed(Y, c.V, c.theta, c.Pliq, c.Qliq, is_q,
+                                       {1: [(1, True, c.Qmin[0] - c.Qload[1])], 2: [(1, False, 0.0)]})
+    assert rec[2]["V"][1] < c.Vset[0] - folga
+    Vkeep = rec[2]["V"][1]
+    # the oracle takes the switches at exactly those passes
+    events = []
+    for k in range(1, 6):
+        r = c.solve(tol=1e-8, lim=True, max_iter=k)
+        events.append((r["n_switch_events"], bool(r["limQ"][0])))
+    assert events == [(0, False), (1, True), (2, False), (2, False), (2, False)]
+    r = c.solve(tol=1e-8, lim=True)
+    assert r["converged"] and r["iterations"] == itp == 4
+    assert not r["limQ"][0] and r["n_switch_events"] == 2
+    assert abs(r["V"][1] - Vkeep) <= 1e-9 and abs(r["V"][1] - c.Vset[0]) > 5e-3
+    assert np.abs(r["V"] - Vp).max() <= 1e-9 and np.abs(r["theta"] - thp).max() <= 1e-9
 
 
 # ------------------------------------------- reactive-limit rules by hand

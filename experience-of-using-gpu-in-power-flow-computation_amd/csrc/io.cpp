@@ -77,7 +77,9 @@ pf_status read_case(const std::string &path, double v0, double theta0, pf_networ
     bool wide;
     if (ext == "cdf") wide = false;
     else if (ext == "cdfx") wide = true;
-    else return PF_INVALID_ARG;   // .pwf / .m readers are outside this library
+    else if (ext == "m") return read_matpower(path, v0, theta0, net);     // CPU/leitura/arquivo.cpp:43-47
+    else if (ext == "pwf") return read_pwf(path, v0, theta0, net);        // CPU/leitura/arquivo.cpp:37-41
+    else return PF_INVALID_ARG;
     const size_t o = wide ? 1 : 0, ob = wide ? 2 : 0;
 
     std::ifstream f(path);

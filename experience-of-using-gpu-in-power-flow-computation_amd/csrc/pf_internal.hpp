@@ -147,7 +147,10 @@ void set_last_error(const std::string &what);
 void options_default(pf_options &o);
 pf_status options_parse(const std::string &text, int cpu_layout, pf_options &o);
 
-// CDF / CDFX reader
+// CDF / CDFX reader, dispatching .m (MATPOWER) and .pwf (ANAREDE) files by
+// extension (CPU/leitura/arquivo.cpp:3-51)
 pf_status read_case(const std::string &path, double v0, double theta0, pf_network &net);
+pf_status read_matpower(const std::string &path, double v0, double theta0, pf_network &net);
+pf_status read_pwf(const std::string &path, double v0, double theta0, pf_network &net);
 
 }  // namespace pf

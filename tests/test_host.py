@@ -221,3 +221,81 @@ def test_generator_workloads(pf, synth, cases, tmp_path):
     assert (sum(b.type == 2 for b in c14.buses), sum(b.type == 0 for b in c14.buses), len(c14.branches)) == (4, 9, 20)
     f = synth.load_factors(118, 3000000)
     assert f.min() >= 0.7 and f.max() <= 1.3 and np.array_equal(f, synth.load_factors(118, 3000000))
+
+
+# ------------------------------------------------ MATPOWER and ANAREDE readers
+
+def _net_arrays(net):
+    return {a: np.asarray(getattr(net, a)) for a in ("V", "theta", "Pliq", "Qliq", "Pload", "Qload", "Pg", "Qg",
+                                                      "Vbase", "gsh", "bsh", "pv", "pq", "Qmin", "Qmax", "Vset",
+                                                      "frm", "to", "r", "x", "bch", "tap_re", "tap_im")}
+
+
+def test_reader_matpower_known_answers(pf):
+    """tests/golden/mp5.m read by hand (CPU/leitura/m.cpp:246-536): bus rows
+    in file order, the first generator row of a PV / slack bus, none for a PQ
+    bus or a PV bus without one, shunts over baseMVA, every branch kept."""
+    n = pf.Network(os.path.join(GOLDEN, "mp5.m"), v0=1.0, theta0=0.0)
+    a = _net_arrays(n)
+    assert (n.nB, n.nL, n.nPV, n.nPQ, n.slack) == (5, 6, 2, 2, 0)
+    assert list(a["pv"]) == [1, 3] and list(a["pq"]) == [2, 4]
+    assert np.array_equal(a["V"], [1.06, 1.045, 1.0, 1.01, 1.0]) and np.all(a["theta"] == 0)
+    assert np.array_equal(a["Pload"], [0, 21.7 / 100, 47.8 / 100, 7.6 / 100, 94.2 / 100])
+    assert np.array_equal(a["Qload"], [0, 12.7 / 100, -3.9 / 100, 1.6 / 100, 19 / 100])
+    assert np.array_equal(a["Pg"], [232.4 / 100, 40 / 100, 0, 0, 0])      # bus 5 is PQ: its generator is not read
+    assert np.array_equal(a["Qg"], [-16.9 / 100, 42.4 / 100, 0, 0, 0])
+    assert np.array_equal(a["Pliq"], a["Pg"] - a["Pload"]) and np.array_equal(a["Qliq"], a["Qg"] - a["Qload"])
+    assert np.array_equal(a["Qmax"], [50 / 100, 0]) and np.array_equal(a["Qmin"], [-40 / 100, 0])
+    assert np.array_equal(a["Vset"], [1.045, 1.01])
+    assert np.array_equal(a["gsh"], [0, 0, 0, 0, 5 / 100]) and np.array_equal(a["bsh"], [0, 0, 19 / 100, 0, 0])
+    assert np.array_equal(a["Vbase"], [230, 230, 138, 138, 138])
+    assert list(a["frm"]) == [0, 0, 1, 1, 2, 3] and list(a["to"]) == [1, 4, 2, 2, 3, 4]
+    assert np.array_equal(a["r"], [0.01938, 0.05403, 0.04699, 0.04699, 0.06701, 0.01335])
+    phi = 5 * np.pi / 180
+    assert np.array_equal(a["tap_re"], [1, 1, 0.978 * np.cos(phi), 1, 1, 1])
+    assert np.array_equal(a["tap_im"], [0, 0, 0.978 * np.sin(phi), 0, 0, 0])
+
+
+def test_reader_matpower_ieee14(pf):
+    """ieee14.m (tests/golden/make_reader_fixtures.py restates the CDF text in
+    MATPOWER form) reads as the same network as ieee14.cdf."""
+    a = _net_arrays(pf.Network(os.path.join(IEEE, "ieee14.cdf")))
+    b = _net_arrays(pf.Network(os.path.join(IEEE, "ieee14.m")))
+    for k in a:
+        if k in ("gsh", "bsh"):   # pu in the CDF, MVAr at 1 pu over baseMVA in MATPOWER
+            assert np.allclose(a[k], b[k], rtol=1e-15, atol=1e-17), k
+        elif k != "Vbase":
+            assert np.array_equal(a[k], b[k]), k
+
+
+def test_reader_pwf_known_answers(pf):
+    """tests/golden/pwf5.pwf read by hand (CPU/leitura/pwf.cpp:3-516): fixed
+    columns, implied decimal points, defaults of blank fields, 32-bit record
+    values, BASE 50, the voltage group lookup, a merged repeated branch with
+    the reference's 1 / (B_raw + b) charging, and a DCSC series capacitor."""
+    f32 = np.float32
+    n = pf.Network(os.path.join(GOLDEN, "pwf5.pwf"), v0=1.0, theta0=0.0)
+    a = _net_arrays(n)
+    assert (n.nB, n.nL, n.nPV, n.nPQ, n.slack, n.n_duplicates) == (4, 5, 1, 2, 0, 1)
+    assert list(a["pv"]) == [1] and list(a["pq"]) == [2, 3]
+    assert np.array_equal(a["V"], [float(f32(1.06)), float(f32(1.045)), 1.0, 1.0])
+    assert np.array_equal(a["Pg"], [float(f32(232.4)) / 50, float(f32(40.0)) / 50, 0, 0])
+    assert np.array_equal(a["Qg"], [float(f32(-16.9)) / 50, float(f32(42.4)) / 50, 0, 0])
+    assert np.array_equal(a["Qmin"], [float(f32(-40.0)) / 50]) and np.array_equal(a["Qmax"], [float(f32(50.0)) / 50])
+    assert np.array_equal(a["Pload"], [0, float(f32(21.7)) / 50, float(f32(94.2)) / 50, float(f32(7.6)) / 50])
+    assert np.array_equal(a["Qload"], [0, float(f32(12.7)) / 50, float(f32(19.0)) / 50, float(f32(1.6)) / 50])
+    assert np.array_equal(a["Vbase"], [230, 0, 138, 138])   # group " B" is not the DGBT key "B"
+    assert np.all(a["gsh"] == 0) and np.all(a["bsh"] == 0)
+    assert list(a["frm"]) == [0, 0, 1, 2, 3] and list(a["to"]) == [1, 2, 2, 3, 0]
+
+    def pct(v):   # float record value over an int 100, in float
+        return float(f32(v) / f32(100))
+    z1 = complex(0.0, pct(4.2))
+    z2 = complex(pct(1.0), pct(4.2))
+    zm = z1 * z2 / (z1 + z2)
+    assert np.allclose(a["r"], [pct(1.938), pct(5.4), zm.real, pct(1.335), 0.0], rtol=1e-15, atol=0)
+    assert np.allclose(a["x"], [pct(5.917), pct(22.3), zm.imag, pct(4.211), pct(-2.5)], rtol=1e-15, atol=0)
+    assert np.array_equal(a["bch"], [float(f32(5.28)) / 50, float(f32(4.92)) / 50, 1.0, float(f32(1.28)) / 50, 0])
+    ph = float(f32(-3.0)) * np.pi / 180
+    assert np.array_equal(a["tap_re"], [1, 1, float(f32(0.978)) * np.cos(ph), 1, 1])
+    assert np.array_equal(a["tap_im"], [0, 0, float(f32(0.978)) * np.sin(ph), 0, 0])

@@ -85,8 +85,11 @@ def test_reader_errors(pf, tmp_path):
     with pytest.raises(pf.PfError) as e:
         pf.Network(str(tmp_path / "missing.cdf"))
     assert e.value.status == pf.PF_IO_ERROR
-    with pytest.raises(pf.PfError) as e:
+    with pytest.raises(pf.PfError) as e:   # .pwf and .m are read, other extensions are not
         pf.Network(str(tmp_path / "case.pwf"))
+    assert e.value.status == pf.PF_IO_ERROR
+    with pytest.raises(pf.PfError) as e:
+        pf.Network(write("case.raw", src))
     assert e.value.status == pf.PF_INVALID_ARG
 
 

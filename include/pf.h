@@ -56,7 +56,12 @@ typedef enum pf_status {
 typedef enum pf_metodo {
     PF_ESPARSO = 0,         /* GPU Jacobian + GPU sparse LU                           */
     PF_DENSO = 1,           /* GPU Jacobian + GPU dense LU with partial pivoting      */
-    PF_HIBRIDO_C = 2        /* GPU Jacobian + host sparse LU (the reference: PARDISO) */
+    PF_HIBRIDO_C = 2,       /* GPU Jacobian + host sparse LU (the reference: PARDISO) */
+    PF_HIBRIDO_A = 3,       /* the reference's Ybus-to-CSR-on-GPU variant: here the sparse
+                               GPU path (the reference takes no linear solve for it,
+                               GPU/newtonRaphson.cuh:452-535; this library solves J dx = g) */
+    PF_HIBRIDO_B = 4        /* GPU sparse Jacobian, made dense, GPU dense LU
+                               (GPU/newtonRaphson.cuh:480-530): the denso path here     */
 } pf_metodo;
 
 typedef enum pf_precision {
@@ -263,6 +268,26 @@ pf_status pf_batch_get(pf_batch *b, double *V, double *theta, int *iterations,
  * iterativo->limQ, CPU/limInjReat.cpp:38-91) and the PV<->PQ flips of that
  * solve (flat-start fallback: of the fallback solve).  Either may be NULL.  */
 pf_status pf_batch_get_limq(pf_batch *b, unsigned char *limQ, int *flips);
+/* Allocation-free batched solve.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Solves n_scen scenarios of
+ * the batch's networks with every input and output in caller device memory,
+ * packed as in pf_batch_set_scenarios / pf_batch_get / pf_batch_get_limq:
+ * net_of [n_scen], Pliq / Qliq / Qload (and optionally V0 / theta0) [sum nB],
+ * V / theta [sum nB], iterations / converged / flips [n_scen],
+ * max_mismatch [n_scen], limQ [sum nPV].  Per-call tables, work counters and
+ * the per-workgroup working-set slices live in a fixed device array of
+ * 64 MiB per GPU (held until the process ends); a call whose tables and
+ * slices do not fit is refused with PF_INVALID_ARG.  The call allocates
+ * nothing.  stream NULL: the batch's own stream, and the call waits for the
+ * results (an index outside the networks then gives PF_INVALID_ARG);
+ * otherwise a hipStream_t: the work is queued there and not waited for.
+ * Calls that overlap on one GPU race on the fixed array.                    */
+pf_status pf_batch_solve_noalloc(pf_batch *b, int n_scen, const int *net_of, const double *Pliq,
+                                 const double *Qliq, const double *Qload, const double *V0,
+                                 const double *theta0, double *V, double *theta, int *iterations,
+                                 int *converged, double *max_mismatch, unsigned char *limQ, int *flips,
+                                 void *stream);
 /* Autotuner (no counterpart in the reference): for every topology of the
  * current scenario set, times one launch of its persistent kernel for each
  * candidate launch shape (working set fully in LDS / step vectors and level

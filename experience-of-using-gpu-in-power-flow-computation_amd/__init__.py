@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpf.so")
 
 PF_OK, PF_NOT_CONVERGED, PF_SINGULAR, PF_PARSE_ERROR, PF_DEVICE_ERROR, PF_INVALID_ARG, PF_IO_ERROR = range(7)
-ESPARSO, DENSO, HIBRIDO_C = 0, 1, 2
+ESPARSO, DENSO, HIBRIDO_C, HIBRIDO_A, HIBRIDO_B = 0, 1, 2, 3, 4
 FP64, MIXED = 0, 1
 BUS_FIELDS = ["V", "theta", "Pliq", "Qliq", "Pload", "Qload", "Pg", "Qg", "Vbase", "gsh", "bsh"]
 

@@ -119,7 +119,7 @@ std::string case_stem(const std::string &path) {
 // benchmarksPrintFile (CPU/impressao.cpp:258-419): a two-line header when the
 // file is new, then one row per run numbered by the rows already there
 void write_csv(const pf_options &opt, const pf_timing &t, double total_ms, int iterations) {
-    static const char *met[] = {"esparso", "denso", "hibridoC"};
+    static const char *met[] = {"esparso", "denso", "hibridoC", "hibridoA", "hibridoB"};
     const std::string name = std::string("out_") + (opt.precision == PF_MIXED ? "mixed_" : "double_") +
                               met[opt.metodo] + "-" + case_stem(opt.arq_entrada) + ".csv";
     unsigned run = 1;
@@ -222,7 +222,7 @@ int main(int argc, char **argv) {
     tr.messages = opt.output_iter && !opt.laconic_mode;
     tr.max_line = (opt.verbose_mode || opt.output_iter) && !opt.laconic_mode;
     pf_solver_set_trace(s, on_pass, &tr);
-    static const char *names[] = {"ESPARSO", "DENSO", "HIBRIDO_C"};
+    static const char *names[] = {"ESPARSO", "DENSO", "HIBRIDO C", "HIBRIDO A", "HIBRIDO B"};   // GPU/main.cu:35-60
     if (opt.laconic_mode) std::printf("[%s]\n", names[opt.metodo]);
     else std::printf("\n\n[%s]\n", names[opt.metodo]);
     pf_result r;

@@ -250,7 +250,9 @@ pf_status options_parse(const std::string &s, int cpu_layout, pf_options &o) {
     if (v == "esparso") o.metodo = PF_ESPARSO;
     else if (v == "denso") o.metodo = PF_DENSO;
     else if (v == "hibridoC") o.metodo = PF_HIBRIDO_C;
-    else return PF_INVALID_ARG;    // hibridoA/B, esparsoSimples, denso_LAPACKE: not provided
+    else if (v == "hibridoA") o.metodo = PF_HIBRIDO_A;   // GPU/sistInfo.h:103-110
+    else if (v == "hibridoB") o.metodo = PF_HIBRIDO_B;
+    else return PF_INVALID_ARG;    // esparsoSimples, denso_LAPACKE: not provided
     if (!value_of(s, "lim_inj_reat", 2, v)) return PF_PARSE_ERROR;
     if (v == "OFF") o.lim_inj_reat = 0;
     else if (v == "ON") o.lim_inj_reat = 1;

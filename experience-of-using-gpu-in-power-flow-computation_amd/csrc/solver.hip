@@ -127,7 +127,12 @@ struct pf_solver {
     }
 
     int lim() const { return opt.lim_inj_reat ? 1 : 0; }
-    bool mixed() const { return opt.precision == PF_MIXED && opt.metodo == PF_ESPARSO; }
+    // the solve path of the metodo: hibridoA takes the sparse GPU path,
+    // hibridoB the GPU Jacobian + dense GPU LU path (GPU/newtonRaphson.cuh:480-530)
+    int path() const {
+        return opt.metodo == PF_HIBRIDO_A ? PF_ESPARSO : opt.metodo == PF_HIBRIDO_B ? PF_DENSO : opt.metodo;
+    }
+    bool mixed() const { return opt.precision == PF_MIXED && path() == PF_ESPARSO; }
 
     pf_status build();
     pf_status reset_state();
@@ -405,7 +410,7 @@ pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
 // factor at the current state and solve J x = g (x, in node order)
 pf_status pf_solver::enqueue_factor_solve(int gate) {
     const int m = 2 * sym.n;
-    if (opt.metodo == PF_DENSO) {
+    if (path() == PF_DENSO) {
         PFCHK(ensure_dense());
         mark(2);
         k_jfill<double><<<(sym.nslot() + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk64, gate);
@@ -484,7 +489,7 @@ pf_status pf_solver::account_pass(bool step, double *pass_ms) {
     pass_ms[0] = el(0, 1);
     if (step) {
         pass_ms[1] = el(2, 3);
-        pass_ms[2] = opt.metodo == PF_HIBRIDO_C ? hybrid_linear_ms : el(3, 4);
+        pass_ms[2] = path() == PF_HIBRIDO_C ? hybrid_linear_ms : el(3, 4);
         pass_ms[3] = el(4, 5);
     }
     times.pq_ms += pass_ms[0];
@@ -537,7 +542,7 @@ pf_status pf_solver::solve(pf_result *res) {
     HIPCHK(hipGetLastError());
     const int maxit = opt.no_max_iter;
     Ctl c{};
-    if (opt.use_graph && opt.metodo == PF_ESPARSO && !timing && !trace_cb) {
+    if (opt.use_graph && path() == PF_ESPARSO && !timing && !trace_cb) {
         if (!gexec) {
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             pf_status st = enqueue_head(1);
@@ -577,7 +582,7 @@ pf_status pf_solver::solve(pf_result *res) {
         double pass_ms[4];
         for (int it = 0; it < maxit; it++) {
             PFCHK(enqueue_head(1));
-            if (opt.metodo == PF_HIBRIDO_C) {
+            if (path() == PF_HIBRIDO_C) {
                 PFCHK(get(&c, ctl, 1));
                 const Ctl head = c;
                 if (c.step) PFCHK(run_hybrid_step());
@@ -614,7 +619,7 @@ pf_status pf_solver_create(const pf_network *net, const pf_options *opt, int dev
     if (!net || !opt || !out) return PF_INVALID_ARG;
     *out = nullptr;
     if (net->nB < 2 || net->slack < 0) return PF_INVALID_ARG;
-    if (opt->metodo < PF_ESPARSO || opt->metodo > PF_HIBRIDO_C) return PF_INVALID_ARG;
+    if (opt->metodo < PF_ESPARSO || opt->metodo > PF_HIBRIDO_B) return PF_INVALID_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
         g_last_error = "no HIP device";
@@ -925,7 +930,7 @@ pf_status pf_stage_lu_solve(pf_solver *s, const double *V, const double *theta,
     s->ctl_host[0] = c;
     PFCHK(s->put(s->ctl, s->ctl_host, 1));
     std::vector<double> xn(2 * (size_t)n);
-    if (s->opt.metodo == PF_HIBRIDO_C) {
+    if (s->path() == PF_HIBRIDO_C) {
         const size_t ns = s->sym.nslot();
         k_jfill<double><<<(ns + 255) / 256, 256, 0, s->stream>>>(s->dn, s->ds, s->dy, s->blk64, 0);
         HIPCHK(hipGetLastError());

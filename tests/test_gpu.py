@@ -283,6 +283,15 @@ def test_nr_hibrido(pf, oracle, cases, name):
     _nr_check(pf, oracle, _path(cases, name), True, pf.HIBRIDO_C)
 
 
+@pytest.mark.parametrize("metodo", ["hibridoA", "hibridoB"])
+@pytest.mark.parametrize("name", ["c14", "c118"])
+def test_nr_hibrido_ab(pf, oracle, cases, name, metodo):
+    """The GPU build's hibridoA (sparse path) and hibridoB (sparse Jacobian
+    made dense, dense LU) solve like the oracle, in the same iterations."""
+    _say("nr", metodo, name)
+    _nr_check(pf, oracle, _path(cases, name), True, pf.HIBRIDO_A if metodo == "hibridoA" else pf.HIBRIDO_B)
+
+
 def test_nr_15104(pf, oracle, cases):
     p = cases.path("c15104")
     _say("nr 15104 fp64 graph")

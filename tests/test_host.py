@@ -120,11 +120,14 @@ def test_options_parse(pf):
     assert o.lim_inj_reat == 0
     o = pf.options(DATA_TXT.format(metodo="denso", extra="precisao = MISTA;\ngrafo = ON;\n"))
     assert o.metodo == pf.DENSO and o.precision == pf.MIXED and o.use_graph == 1
+    # the GPU build's other hybrid names (GPU/sistInfo.h:103-110)
+    assert pf.options(DATA_TXT.format(metodo="hibridoA", extra="")).metodo == pf.HIBRIDO_A
+    assert pf.options(DATA_TXT.format(metodo="hibridoB", extra="")).metodo == pf.HIBRIDO_B
 
 
 def test_options_errors(pf, tmp_path):
     with pytest.raises(pf.PfError) as e:
-        pf.options(DATA_TXT.format(metodo="hibridoA", extra=""))
+        pf.options(DATA_TXT.format(metodo="esparsoSimples", extra=""))
     assert e.value.status == pf.PF_INVALID_ARG
     with pytest.raises(pf.PfError) as e:
         pf.options(DATA_TXT.format(metodo="esparso", extra="").replace("= 37", "= abc"))

@@ -375,6 +375,7 @@ def main():
     ms_step = 1e3 * wall / args.steps
 
     extras = {}
+    spfp_gpu = None
     # roofline of each persistent kernel from its own timed launch
     roof = None
     log("headline batch done", f"{value:.1f} solves/s")
@@ -431,6 +432,12 @@ def main():
                                                 "one captured graph per NR iteration, level-scheduled LU"}
         extras["batched_mixed_gpu_118"] = {"solves_per_s": round(batched118, 1)}
         extras["batched_vs_plain_118"] = round(batched118 / plain, 1)
+        # the mixed-precision answer of the 118-bus case without Q limits, for
+        # the single-precision (SPFP) reference point computed on the host
+        sm = pf.Solver(n118, pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=0, precision=pf.MIXED,
+                                        refine_steps=args.refine, use_graph=1))
+        rm = sm.solve()
+        spfp_gpu = (sm.state()[:2], rm)
 
     if not args.no_extras and not args.no_single:
         log("single 15,104-bus solve")
@@ -507,6 +514,21 @@ def main():
                                         "same_converged_set": bool(np.array_equal(np.concatenate(cg),
                                                                                   np.concatenate([Cb, Ca]))),
                                         "same_limQ_set": same_lq, "scenarios": nsc}
+        # the reference's whole-program single-precision build (oracle/spfp.py)
+        # as an accuracy reference point for the mixed-precision design
+        if spfp_gpu is not None:
+            import spfp
+            o = o118.solve(tol=1e-8, max_iter=60, lim=False)
+            V32, T32, it32, ok32 = spfp.solve(o118, np.float32, tol=1e-4, max_iter=60)
+            (Vm, Tm), rm = spfp_gpu
+            extras["spfp_reference_point"] = {
+                "case": "118-bus, Q limits off, flat start",
+                "spfp": {"tol": 1e-4, "converged": bool(ok32), "iterations": int(it32),
+                         "max_abs_dV_vs_fp64_pu": float(np.abs(V32 - o["V"]).max()),
+                         "max_abs_dtheta_vs_fp64_rad": float(np.abs(T32 - o["theta"]).max())},
+                "mixed": {"tol": 1e-8, "converged": bool(rm["converged"]), "iterations": int(rm["iterations"]),
+                          "max_abs_dV_vs_fp64_pu": float(np.abs(Vm - o["V"]).max()),
+                          "max_abs_dtheta_vs_fp64_rad": float(np.abs(Tm - o["theta"]).max())}}
         # warm-started answers (Q limits off) against the flat-start oracle
         for path, chk in warm_checks.items():
             oc = O.Case(path)

@@ -536,3 +536,23 @@ def test_bf16_rounding_known_answers():
     assert M.bf16(np.float32(1 + 3 * 2 ** -8)) == 1 + 2 ** -6
     assert M.bf16(np.float32(1 + 2 ** -7)) == 1 + 2 ** -7
     assert M.bf16(np.float32(-3.0)) == -3.0
+
+
+# ------------------------------------------------ whole-program FP32 (SPFP)
+
+@pytest.mark.parametrize("name", ["ieee14", "c118"])
+def test_spfp_restatement(oracle, name):
+    """oracle/spfp.py (the reference's single-precision build restated):
+    with dtype float64 it is an FP64 NR and reproduces the oracle without
+    limits (1e-9); with float32 it converges at the reference's default
+    tolerance 1e-4 (CPU/global.cpp:10) to within FP32 accuracy of it."""
+    import spfp
+    c = oracle.Case(case_path(name))
+    o = c.solve(tol=1e-8, lim=False)
+    V64, th64, it64, ok64 = spfp.solve(c, np.float64, tol=1e-8)
+    assert ok64 and it64 == o["iterations"]
+    assert np.abs(V64 - o["V"]).max() <= 1e-9 and np.abs(th64 - o["theta"]).max() <= 1e-9
+    V32, th32, it32, ok32 = spfp.solve(c, np.float32, tol=1e-4)
+    assert ok32
+    err = max(np.abs(V32 - o["V"]).max(), np.abs(th32 - o["theta"]).max())
+    assert 1e-9 < err < 1e-3

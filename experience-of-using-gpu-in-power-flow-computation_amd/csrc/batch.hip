@@ -75,6 +75,13 @@ template <int MODE> constexpr int batch_chain_cap() { return MODE == 2 ? kBatchC
 // the whole workgroup on the level's destinations.  A constant, not the
 // number of lane groups, so that every workgroup size sums in the same order.
 constexpr int kWideLevel = 32;
+// the narrow levels at the top of the tree (below the dense block), when
+// their update pairs are chunked: chunks of kBatchChunkPairs pairs, one lane
+// group of kBatchChunkLanes per chunk (one pair per lane), at most
+// kBatchMaxChunks chunks on a level (LDS partial sums; kernels whose working
+// set lives in global memory only)
+constexpr int kBatchChunkPairs = 8, kBatchChunkLanes = 8, kBatchMaxChunks = 512;
+static_assert(kWideLevel <= 4 * kBatchChain, "the narrow-level pivots pass through the dense-top LDS buffer");
 // update pairs loaded per round in the factorisation (registers are tight
 // in the persistent kernel)
 constexpr int kBatchUnroll = 1;
@@ -172,13 +179,125 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
     }
 }
 
+// Narrow elimination-tree levels [l0, l1) (fewer than kWideLevel nodes
+// each) by the whole workgroup, two barriers per level, with the same
+// arithmetic as coop_factor_range: groups of G lanes take the level's
+// destination blocks, the lanes of a group split the destination's update
+// pairs (pair q + G*i for lane q) and reduce them in a fixed order; after a
+// barrier the pivot's inverse is stored and each column block is scaled by
+// it.  What is loaded is arranged for latency: the indices of a group's
+// first destination on the next level and of each lane's first pair are
+// loaded one level ahead, a destination's current blocks are loaded with its
+// first pair's blocks, the updated pivots pass through LDS (s_piv, one per
+// node of the level) and the first updated column block stays in registers,
+// so the second half of a level waits on no global load.
+template <typename T, int G>
+__device__ void wg_factor_narrow(const DevSym &Y, T *blk, int l0, int l1, int *bad, B4<T> *s_piv) {
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
+    const int grp = (int)threadIdx.x / G, lane = (int)threadIdx.x % G, ng = (int)blockDim.x / G;
+    struct Dest {
+        int j, t, q, qe, pa, pb, p;   // p: the column block's position (t > 0)
+    };
+    auto dest = [&](int e0, int e) {
+        Dest r;
+        r.j = Y.ld_node[e0 + e];
+        const int d = Y.ld_d[e0 + e];
+        r.t = d - Y.fd_ptr[r.j];
+        r.q = Y.fc_ptr[d] + lane;
+        r.qe = Y.fc_ptr[d + 1];
+        r.pa = r.pb = 0;
+        if (r.q < r.qe) {
+            r.pa = Y.fc_a[r.q];
+            r.pb = Y.fc_b[r.q];
+        }
+        r.p = r.t ? Y.colptr[r.j] + r.t - 1 : 0;
+        return r;
+    };
+    Dest cur{};
+    cur.j = -1;
+    if (l0 < l1 && grp < Y.ld_ptr[l0 + 1] - Y.ld_ptr[l0]) cur = dest(Y.ld_ptr[l0], grp);
+    for (int l = l0; l < l1; l++) {
+        const int e0 = Y.ld_ptr[l], ne = Y.ld_ptr[l + 1] - e0, s0 = Y.lvl_ptr[l];
+        Dest nxt{};
+        nxt.j = -1;
+        if (l + 1 < l1 && grp < Y.ld_ptr[l + 2] - Y.ld_ptr[l + 1]) nxt = dest(Y.ld_ptr[l + 1], grp);
+        B4<T> keep{0, 0, 0, 0};   // the first destination's updated column block
+        int keep_lrp = 0;
+        for (int e = grp; e < ne; e += ng) {
+            const Dest D = (e == grp) ? cur : dest(e0, e);
+            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+            // the destination's own blocks, loaded with its first pair's
+            B4<T> v0{0, 0, 0, 0}, v1{0, 0, 0, 0};
+            if (lane == 0) {
+                v0 = load_b4(blk, D.t ? L0 + D.p : D.j);
+                if (D.t) v1 = load_b4(blk, U0 + D.p);
+            }
+            const int lrp = (lane == 0 && D.t) ? Y.lr_pos[D.p] : 0;
+            if (D.q < D.qe) {
+                mma2(a, load_b4(blk, L0 + D.pa), load_b4(blk, U0 + D.pb));
+                if (D.t != 0) mma2(u, load_b4(blk, L0 + D.pb), load_b4(blk, U0 + D.pa));
+            }
+            if (D.t == 0) pairs_acc<T, false, 1>(Y, blk, D.q + G, D.qe, G, a, u);
+            else pairs_acc<T, true, 1>(Y, blk, D.q + G, D.qe, G, a, u);
+            a = group_sum<T, G>(a);
+            u = group_sum<T, G>(u);
+            if (lane == 0) {
+                if (D.t == 0) {
+                    const B4<T> P{v0.a - a.a, v0.b - a.b, v0.c - a.c, v0.d - a.d};
+                    store_b4(blk, D.j, P);
+                    s_piv[D.j - s0] = P;
+                } else {
+                    const B4<T> Lb{v0.a - a.a, v0.b - a.b, v0.c - a.c, v0.d - a.d};
+                    store_b4(blk, L0 + D.p, Lb);
+                    store_b4(blk, U0 + D.p, B4<T>{v1.a - u.a, v1.b - u.b, v1.c - u.c, v1.d - u.d});
+                    if (e == grp) {
+                        keep = Lb;
+                        keep_lrp = lrp;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int e = grp; e < ne; e += ng) {
+                int j, t, p, lrp;
+                B4<T> Lb;
+                if (e == grp) {
+                    j = cur.j; t = cur.t; p = cur.p; lrp = keep_lrp; Lb = keep;
+                } else {
+                    j = Y.ld_node[e0 + e];
+                    t = Y.ld_d[e0 + e] - Y.fd_ptr[j];
+                    p = t ? Y.colptr[j] + t - 1 : 0;
+                    lrp = t ? Y.lr_pos[p] : 0;
+                    Lb = t ? load_b4(blk, L0 + p) : B4<T>{0, 0, 0, 0};
+                }
+                const B4<T> P = s_piv[j - s0];
+                T det = P.a * P.d - P.b * P.c;
+                const bool sing = !(det != (T)0) || !isfinite((double)det);
+                if (sing) det = (T)1;
+                const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
+                if (t == 0) {
+                    if (sing) atomicOr(bad, 1);
+                    store_b4(blk, DI + j, Pi);
+                } else {
+                    const B4<T> lb = mul(Lb, Pi);
+                    store_b4(blk, L0 + p, lb);
+                    store_b4(blk, LR0 + lrp, lb);
+                }
+            }
+        }
+        __syncthreads();
+        cur = nxt;
+    }
+}
+
 // Forward and backward sweeps, GL lanes per node.  The first entry of the
 // first node a lane group takes on a level (index and factor block, which do
 // not depend on x) is loaded one level ahead, so its latency overlaps the
 // previous level's arithmetic and barrier; the summation order is the same
 // as without it.
 template <typename T>
-__device__ void wg_solve(const DevSym &Y, const T *blk, double *x, double *s_r) {
+__device__ __forceinline__ void wg_solve(const DevSym &Y, const T *blk, double *x, double *s_r) {
     const int grp = (int)threadIdx.x / GL, lane = (int)threadIdx.x % GL, ng = (int)blockDim.x / GL;
     const int LR0 = Y.n + 2 * Y.nnzL, U0 = Y.n + Y.nnzL;
     const int nl = Y.nlev_s;
@@ -404,9 +523,9 @@ __device__ void wg_set_E(const DevNet &N, Work<T> &w) {
 // One NR solve of one scenario by the whole workgroup.  Returns (via shared
 // result) iterations, converged flag and final max mismatch.
 template <typename T>
-__device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const double *Pliq,
+__device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const double *Pliq,
                       const double *Qliq, const double *Qload, const double *V0, const double *th0,
-                      int *s_it, int *s_conv, double *s_maxg, int *s_fl, B4<T> *s_chain) {
+                      int *s_it, int *s_conv, double *s_maxg, int *s_fl, B4<T> *s_chain, B4<T> *s_ck) {
     const DevNet &N = tp.N;
     const DevSym &Y = tp.Y;
     __shared__ unsigned long long s_maxG_bits, s_maxDx_bits;
@@ -519,6 +638,11 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
         {
             const int grp = tid / GL, lane = tid % GL, ng = (int)blockDim.x / GL;
             for (int l = 0; l < Y.nlev_s;) {
+                if (s_ck && Y.ck_lvl && l >= Y.tail_start) {   // the chunked narrow top
+                    chunk_factor_levels<T, kBatchChunkLanes, 1>(Y, w.blk, l, Y.nlev_s, &s_sing, s_ck,
+                                                                  s_ck + kBatchMaxChunks, s_chain);
+                    break;
+                }
                 const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
                 if (c >= kWideLevel) {
                     for (int i = grp; i < c; i += ng) wg_factor_node(Y, w.blk, (s + i), lane, &s_sing);
@@ -527,7 +651,7 @@ __device__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Work<T> &w, const 
                 } else {   // a run of narrow levels
                     int l1 = l + 1;
                     while (l1 < Y.nlev_s && Y.lvl_ptr[l1 + 1] - Y.lvl_ptr[l1] < kWideLevel) l1++;
-                    coop_factor_range<T, kCoopLanes, kBatchUnroll>(Y, w.blk, l, l1, &s_sing);
+                    wg_factor_narrow<T, kCoopLanes>(Y, w.blk, l, l1, &s_sing, s_chain);
                     l = l1;
                 }
             }
@@ -594,11 +718,18 @@ __host__ __device__ inline long long partial_lds_bytes(int n, int nlev) {
 // MODE 1: working set in a global slice, except the step vectors and the
 //         level / row-structure metadata, staged in LDS (latency of the
 //         level-by-level walk); MODE 0: everything in the global slice.
+// The float LDS-resident kernel at 256 threads keeps three waves per SIMD
+// (three workgroups per CU, within 168 VGPRs): small networks are latency
+// bound, so more resident scenarios per CU beat the few registers it costs.
+template <typename T, int MODE, int MAXT>
+constexpr int batch_min_waves() { return (MODE == 2 && MAXT == 256 && sizeof(T) == 4) ? 3 : 1; }
+
 template <typename T, int MODE, int MAXT>   // MAXT: largest workgroup (register budget)
-__global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
+__global__ void __launch_bounds__(MAXT, (batch_min_waves<T, MODE, MAXT>())) k_batch(BatchTopo tp0, BatchJob J) {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_item, s_it, s_conv, s_fl;
     __shared__ B4<T> s_chain[4 * batch_chain_cap<MODE>()];   // dense top: row / column staging
+    __shared__ B4<T> s_ck[MODE == 2 ? 1 : 2 * kBatchMaxChunks];  // chunk partial sums (a, u)
     __shared__ double s_maxg;
     BatchTopo tp = tp0;
     const DevNet &N = tp.N;
@@ -634,13 +765,17 @@ __global__ void __launch_bounds__(MAXT) k_batch(BatchTopo tp0, BatchJob J) {
         const int sc = J.scen[item];
         const long long off = J.off[sc];
         const double *Pl = J.Pliq + off, *Ql = J.Qliq + off, *Qd = J.Qload + off;
-        bool warm = J.V0 != nullptr;
-        wg_nr(tp, J, w, Pl, Ql, Qd, warm ? J.V0 + off : tp.V0flat, warm ? J.th0 + off : tp.th0flat,
-              &s_it, &s_conv, &s_maxg, &s_fl, s_chain);
-        int total_it = s_it;
-        if (warm && !s_conv) {                              // flat-start fallback
-            wg_nr(tp, J, w, Pl, Ql, Qd, tp.V0flat, tp.th0flat, &s_it, &s_conv, &s_maxg, &s_fl, s_chain);
+        const bool warm = J.V0 != nullptr;
+        int total_it = 0;
+        // one call site (the solve is inlined once): the warm start, then,
+        // when it does not converge, the flat-start fallback
+        for (int attempt = warm ? 0 : 1; attempt < 2; ++attempt) {
+            const bool from_warm = attempt == 0;
+            wg_nr(tp, J, w, Pl, Ql, Qd, from_warm ? J.V0 + off : tp.V0flat,
+                  from_warm ? J.th0 + off : tp.th0flat, &s_it, &s_conv, &s_maxg, &s_fl, s_chain,
+                  MODE == 2 ? nullptr : s_ck);
             total_it += s_it;
+            if (s_conv) break;                              // shared: uniform over the workgroup
         }
         for (int k = threadIdx.x; k < N.nB; k += blockDim.x) {
             J.Vout[off + k] = w.V[k];
@@ -882,14 +1017,17 @@ pf_status pf_batch::add_net(const pf_network &src) {
     const long long max_lds = max_lds_i - 8192;   // room for the kernel's static LDS
     b->net = src;
     b->yp = build_ypattern(src);
-    b->sym = analyze(src, b->yp, 0, opt.lu_blocked ? kBatchChain : 0, kTopWidth);
+    // tail_start: the first level of the narrow top (fewer than kWideLevel
+    // nodes per level up to the dense block)
+    b->sym = analyze(src, b->yp, kWideLevel - 1, opt.lu_blocked ? kBatchChain : 0, kTopWidth);
     {   // a working set beyond LDS runs in MODE 1 / 0, whose kernels take a wider dense top
         const bool mx = opt.precision != PF_FP64;
         const long long w0 = mx ? work_bytes<float>(src.nB, b->sym.n, src.nPV, b->sym.nblk())
                                 : work_bytes<double>(src.nB, b->sym.n, src.nPV, b->sym.nblk());
         if (w0 > kLdsLimit && opt.lu_blocked)
-            b->sym = analyze(src, b->yp, 0, kBatchChainWide, kTopWidth);
+            b->sym = analyze(src, b->yp, kWideLevel - 1, kBatchChainWide, kTopWidth);
     }
+    chunk_tail(b->sym, kBatchChunkPairs);
     if (!b->sym.level_ordered()) {
         pf::set_last_error("symbolic analysis: nodes not numbered level by level");
         return PF_INVALID_ARG;
@@ -946,6 +1084,17 @@ pf_status pf_batch::add_net(const pf_network &src) {
     BCHK(upload((int **)&Y.ld_ptr, sym.ld_ptr));
     BCHK(upload((int **)&Y.ld_node, sym.ld_node));
     BCHK(upload((int **)&Y.ld_d, sym.ld_d));
+    Y.tail_start = sym.tail_start;
+    Y.te0 = sym.ld_ptr[sym.tail_start];
+    Y.ck_lvl = Y.ck_q0 = Y.ck_q1 = Y.te_node = Y.te_p = Y.te_ck = nullptr;
+    if (opt.lu_blocked && sym.ck_max <= kBatchMaxChunks && !sym.ck_q0.empty()) {
+        BCHK(upload((int **)&Y.ck_lvl, sym.ck_lvl));
+        BCHK(upload((int **)&Y.ck_q0, sym.ck_q0));
+        BCHK(upload((int **)&Y.ck_q1, sym.ck_q1));
+        BCHK(upload((int **)&Y.te_node, sym.te_node));
+        BCHK(upload((int **)&Y.te_p, sym.te_p));
+        BCHK(upload((int **)&Y.te_ck, sym.te_ck));
+    }
     BCHK(upload((double **)&b->tp.V0flat, net.V));
     BCHK(upload((double **)&b->tp.th0flat, net.theta));
     b->tp.nslot = sym.nslot();

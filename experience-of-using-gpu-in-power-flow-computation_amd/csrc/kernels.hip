@@ -335,75 +335,11 @@ __global__ void __launch_bounds__(256) k_factor_level(DevSym Y, T *blk, Ctl *ctl
 template <typename T>
 __global__ void __launch_bounds__(1024) k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate) {
     if (gate && !ctl->step) return;
-    if (Y.ck_lvl) chunk_factor_range<T, 4, 4, kMaxChunks>(Y, blk, lv0, lv1, &ctl->singular);
-    else coop_factor_range<T, kCoopLanes, 4>(Y, blk, lv0, lv1, &ctl->singular);
-}
-
-// Tail levels with the update pairs cut into chunks of at most
-// kChunkPairs, each inside one destination, so that one round of loads
-// covers a level whatever its longest destination:
-//   1. lane groups sum one chunk each (tree over the group) into LDS;
-//   2. one thread per destination adds its chunks in order and updates the
-//      block;
-//   3. one thread per destination inverts the pivot into the inverse-pivot
-//      region or scales its column block (both L copies).
-// Three barriers per level; the sums are in a fixed order.
-template <typename T, int G, int UNR, int MAXC>
-__device__ void chunk_factor_range(const DevSym &Y, T *blk, int lv0, int lv1, int *bad) {
-    __shared__ B4<T> s_pa[MAXC], s_pu[MAXC];
-    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
-    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / G, lane = tid % G, ng = bd / G;
-    for (int l = lv0; l < lv1; l++) {
-        const int c0 = Y.ck_lvl[l - Y.tail_start], nc = Y.ck_lvl[l - Y.tail_start + 1] - c0;
-        for (int c = grp; c < nc; c += ng) {
-            const int q0 = Y.ck_q0[c0 + c], q1f = Y.ck_q1[c0 + c], q1 = q1f & kChunkMask;
-            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
-            if (q1f & kChunkDiag) pairs_acc<T, false, UNR>(Y, blk, q0 + lane, q1, G, a, u);
-            else pairs_acc<T, true, UNR>(Y, blk, q0 + lane, q1, G, a, u);
-            a = group_sum<T, G>(a);
-            u = group_sum<T, G>(u);
-            if (lane == 0) {
-                s_pa[c] = a;
-                s_pu[c] = u;
-            }
-        }
-        __syncthreads();
-        const int e0 = Y.ld_ptr[l] - Y.te0, ne = Y.ld_ptr[l + 1] - Y.ld_ptr[l];
-        for (int e = tid; e < ne; e += bd) {
-            const int E = e0 + e, j = Y.te_node[E], p = Y.te_p[E], cb = Y.te_ck[E], ce = Y.te_ck[E + 1];
-            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
-            for (int c = cb; c < ce; c++) {
-                const B4<T> x = s_pa[c - c0], y = s_pu[c - c0];
-                a = B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
-                u = B4<T>{u.a + y.a, u.b + y.b, u.c + y.c, u.d + y.d};
-            }
-            if (p < 0) {
-                const B4<T> P = load_b4(blk, j);
-                store_b4(blk, j, B4<T>{P.a - a.a, P.b - a.b, P.c - a.c, P.d - a.d});
-            } else {
-                const B4<T> Lb = load_b4(blk, L0 + p), Ub = load_b4(blk, U0 + p);
-                store_b4(blk, L0 + p, B4<T>{Lb.a - a.a, Lb.b - a.b, Lb.c - a.c, Lb.d - a.d});
-                store_b4(blk, U0 + p, B4<T>{Ub.a - u.a, Ub.b - u.b, Ub.c - u.c, Ub.d - u.d});
-            }
-        }
-        __syncthreads();
-        for (int e = tid; e < ne; e += bd) {
-            const int E = e0 + e, j = Y.te_node[E], p = Y.te_p[E];
-            const B4<T> P = load_b4(blk, j);
-            T det = P.a * P.d - P.b * P.c;
-            const bool sing = !(det != (T)0) || !isfinite((double)det);
-            if (sing) det = (T)1;
-            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
-            if (p < 0) {
-                if (sing) atomicOr(bad, 1);
-                store_b4(blk, DI + j, Pi);
-            } else {
-                const B4<T> lb = mul(load_b4(blk, L0 + p), Pi);
-                store_b4(blk, L0 + p, lb);
-                store_b4(blk, LR0 + Y.lr_pos[p], lb);
-            }
-        }
-        __syncthreads();
+    if (Y.ck_lvl) {
+        __shared__ B4<T> s_pa[kMaxChunks], s_pu[kMaxChunks], s_piv[kTailWidth];
+        chunk_factor_levels<T, 4, sizeof(T) == 4 ? 4 : 2>(Y, blk, lv0, lv1, &ctl->singular, s_pa, s_pu, s_piv);
+    } else {
+        coop_factor_range<T, kCoopLanes, 4>(Y, blk, lv0, lv1, &ctl->singular);
     }
 }
 
@@ -652,6 +588,182 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
     }
 }
 
+// ------------------------------------------ tail sweeps with x in LDS
+// The tail's unknowns live in LDS for the whole level walk, so a level costs
+// a barrier and LDS round trips instead of dependent reads through the
+// cache hierarchy.  Entries on nodes outside the tail (final before the
+// sweep) are summed first by a grid-wide pass, 16 lanes per tail node.
+
+__device__ __forceinline__ void sum16(double &a0, double &a1) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, 16);
+        a1 += __shfl_xor(a1, off, 16);
+    }
+}
+
+// forward rows of the tail nodes: x_j -= sum over entries on nodes < t0
+template <typename T>
+__global__ void __launch_bounds__(256) k_tail_fwd_ext(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    const int a = (int)blockIdx.x * 16 + (int)threadIdx.x / 16, lane = (int)threadIdx.x & 15;
+    if (a >= D.t1 - D.t0) return;                        // whole 16-lane groups leave together
+    const int LR0 = Y.n + 2 * Y.nnzL;
+    double a0 = 0, a1 = 0;
+    for (int e = D.fx_ptr[a] + lane; e < D.fx_ptr[a + 1]; e += 16) {
+        const B4<T> L = load_b4(blk, LR0 + D.fx_q[e]);
+        const int k = D.fx_k[e];
+        const double x0 = x[2 * k], x1 = x[2 * k + 1];
+        a0 += (double)L.a * x0 + (double)L.b * x1;
+        a1 += (double)L.c * x0 + (double)L.d * x1;
+    }
+    sum16(a0, a1);
+    if (lane == 0) {
+        const int j = D.t0 + a;
+        x[2 * j] -= a0;
+        x[2 * j + 1] -= a1;
+    }
+}
+
+// backward columns of the tail nodes: x_j -= sum over entries on the dense
+// top (nodes >= t1, solved by the forward sweep's end)
+template <typename T>
+__global__ void __launch_bounds__(256) k_tail_bwd_ext(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    const int a = (int)blockIdx.x * 16 + (int)threadIdx.x / 16, lane = (int)threadIdx.x & 15;
+    if (a >= D.t1 - D.t0) return;
+    const int U0 = Y.n + Y.nnzL;
+    double a0 = 0, a1 = 0;
+    for (int e = D.bx_ptr[a] + lane; e < D.bx_ptr[a + 1]; e += 16) {
+        const B4<T> U = load_b4(blk, U0 + D.bx_p[e]);
+        const int i = D.bx_i[e];
+        const double x0 = x[2 * i], x1 = x[2 * i + 1];
+        a0 += (double)U.a * x0 + (double)U.b * x1;
+        a1 += (double)U.c * x0 + (double)U.d * x1;
+    }
+    sum16(a0, a1);
+    if (lane == 0) {
+        const int j = D.t0 + a;
+        x[2 * j] -= a0;
+        x[2 * j + 1] -= a1;
+    }
+}
+
+// LDS layout of one sweep (tail_sweep_lds gives its size): x of the tail
+// nodes, [inverse pivots], [entry blocks], level bounds, entry pointers,
+// entry local indices, [entry slots]
+template <typename T, bool STAGED>
+struct TailLds {
+    double2 *x;
+    B4<T> *dg, *eb;
+    int *lp, *ptr, *ek, *es;
+    __device__ TailLds(char *smem, int nt, int nlt, int ne, bool backward) {
+        x = reinterpret_cast<double2 *>(smem);
+        B4<T> *b = reinterpret_cast<B4<T> *>(x + nt);
+        dg = b;
+        if (backward) b += nt;
+        eb = b;
+        if (STAGED) b += ne;
+        lp = reinterpret_cast<int *>(b);
+        ptr = lp + nlt + 1;
+        ek = ptr + nt + 1;
+        es = ek + ne;
+    }
+};
+
+// levels [lv0, lv1) of the tail, forward: 16 lanes per node, then the dense
+// top (chain_solve) from the finished x
+template <typename T, bool STAGED>
+__global__ void __launch_bounds__(1024) k_fwd_tail_lds(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int lv0,
+                                                       int lv1, int gate) {
+    if (gate && !ctl->step) return;
+    extern __shared__ __align__(16) char smem[];
+    const int nt = D.t1 - D.t0, nlt = lv1 - lv0, ne = D.nfi, t0 = D.t0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / 16, lane = tid & 15, ng = bd / 16;
+    const int LR0 = Y.n + 2 * Y.nnzL;
+    TailLds<T, STAGED> S(smem, nt, nlt, ne, false);
+    for (int a = tid; a < nt; a += bd) S.x[a] = make_double2(x[2 * (t0 + a)], x[2 * (t0 + a) + 1]);
+    for (int l = tid; l <= nlt; l += bd) S.lp[l] = Y.lvl_ptr[lv0 + l] - t0;
+    for (int a = tid; a <= nt; a += bd) S.ptr[a] = D.fi_ptr[a];
+    for (int e = tid; e < ne; e += bd) {
+        S.ek[e] = D.fi_k[e];
+        if constexpr (STAGED) store_b4(reinterpret_cast<T *>(S.eb), e, load_b4(blk, LR0 + D.fi_q[e]));
+        else S.es[e] = LR0 + D.fi_q[e];
+    }
+    __syncthreads();
+    for (int l = 0; l < nlt; l++) {
+        const int s = S.lp[l], c = S.lp[l + 1] - s;
+        for (int idx = grp; idx < c; idx += ng) {
+            const int a = s + idx;
+            double a0 = 0, a1 = 0;
+            for (int e = S.ptr[a] + lane; e < S.ptr[a + 1]; e += 16) {
+                const B4<T> L = load_b4(STAGED ? reinterpret_cast<const T *>(S.eb) : blk, STAGED ? e : S.es[e]);
+                const double2 xk = S.x[S.ek[e]];
+                a0 += (double)L.a * xk.x + (double)L.b * xk.y;
+                a1 += (double)L.c * xk.x + (double)L.d * xk.y;
+            }
+            sum16(a0, a1);
+            if (lane == 0) S.x[a] = make_double2(S.x[a].x - a0, S.x[a].y - a1);
+        }
+        __syncthreads();
+    }
+    for (int a = tid; a < nt; a += bd) {
+        x[2 * (t0 + a)] = S.x[a].x;
+        x[2 * (t0 + a) + 1] = S.x[a].y;
+    }
+    __syncthreads();
+    __shared__ double s_r[2 * kChainMax];
+    if (Y.ch_m > 0) chain_solve<T, 16>(Y, blk, x, s_r);
+}
+
+// levels [lv0, lv1) of the tail, backward, after k_tail_bwd_ext
+template <typename T, bool STAGED>
+__global__ void __launch_bounds__(1024) k_bwd_tail_lds(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int lv0,
+                                                       int lv1, int gate) {
+    if (gate && !ctl->step) return;
+    extern __shared__ __align__(16) char smem[];
+    const int nt = D.t1 - D.t0, nlt = lv1 - lv0, ne = D.nbi, t0 = D.t0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / 16, lane = tid & 15, ng = bd / 16;
+    const int U0 = Y.n + Y.nnzL;
+    TailLds<T, STAGED> S(smem, nt, nlt, ne, true);
+    for (int a = tid; a < nt; a += bd) {
+        S.x[a] = make_double2(x[2 * (t0 + a)], x[2 * (t0 + a) + 1]);
+        store_b4(reinterpret_cast<T *>(S.dg), a, load_b4(blk, Y.di0 + t0 + a));
+    }
+    for (int l = tid; l <= nlt; l += bd) S.lp[l] = Y.lvl_ptr[lv0 + l] - t0;
+    for (int a = tid; a <= nt; a += bd) S.ptr[a] = D.bi_ptr[a];
+    for (int e = tid; e < ne; e += bd) {
+        S.ek[e] = D.bi_i[e];
+        if constexpr (STAGED) store_b4(reinterpret_cast<T *>(S.eb), e, load_b4(blk, U0 + D.bi_p[e]));
+        else S.es[e] = U0 + D.bi_p[e];
+    }
+    __syncthreads();
+    for (int l = nlt - 1; l >= 0; l--) {
+        const int s = S.lp[l], c = S.lp[l + 1] - s;
+        for (int idx = grp; idx < c; idx += ng) {
+            const int a = s + idx;
+            double a0 = 0, a1 = 0;
+            for (int e = S.ptr[a] + lane; e < S.ptr[a + 1]; e += 16) {
+                const B4<T> U = load_b4(STAGED ? reinterpret_cast<const T *>(S.eb) : blk, STAGED ? e : S.es[e]);
+                const double2 xi = S.x[S.ek[e]];
+                a0 += (double)U.a * xi.x + (double)U.b * xi.y;
+                a1 += (double)U.c * xi.x + (double)U.d * xi.y;
+            }
+            sum16(a0, a1);
+            if (lane == 0) {
+                const double y0 = S.x[a].x - a0, y1 = S.x[a].y - a1;
+                const B4<T> P = load_b4(reinterpret_cast<const T *>(S.dg), a);
+                S.x[a] = make_double2((double)P.a * y0 + (double)P.b * y1, (double)P.c * y0 + (double)P.d * y1);
+            }
+        }
+        __syncthreads();
+    }
+    for (int a = tid; a < nt; a += bd) {
+        x[2 * (t0 + a)] = S.x[a].x;
+        x[2 * (t0 + a) + 1] = S.x[a].y;
+    }
+}
+
 // ------------------------------------------------------- branch flows
 // Flows as the reference reports them (SpCalcFlux,
 // CPU/esparso/potencia/spCalcFluxPQ.cpp:3-36): for the direction k -> m,
@@ -893,5 +1005,17 @@ template __global__ void k_fwd_tail<double>(DevSym, const double *, double *, Ct
 template __global__ void k_fwd_tail<float>(DevSym, const float *, double *, Ctl *, int, int, int);
 template __global__ void k_bwd_tail<double>(DevSym, const double *, double *, Ctl *, int, int, int);
 template __global__ void k_bwd_tail<float>(DevSym, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_tail_fwd_ext<double>(DevSym, DevTail, const double *, double *, Ctl *, int);
+template __global__ void k_tail_fwd_ext<float>(DevSym, DevTail, const float *, double *, Ctl *, int);
+template __global__ void k_tail_bwd_ext<double>(DevSym, DevTail, const double *, double *, Ctl *, int);
+template __global__ void k_tail_bwd_ext<float>(DevSym, DevTail, const float *, double *, Ctl *, int);
+template __global__ void k_fwd_tail_lds<double, true>(DevSym, DevTail, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_fwd_tail_lds<double, false>(DevSym, DevTail, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_fwd_tail_lds<float, true>(DevSym, DevTail, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_fwd_tail_lds<float, false>(DevSym, DevTail, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_tail_lds<double, true>(DevSym, DevTail, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_tail_lds<double, false>(DevSym, DevTail, const double *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_tail_lds<float, true>(DevSym, DevTail, const float *, double *, Ctl *, int, int, int);
+template __global__ void k_bwd_tail_lds<float, false>(DevSym, DevTail, const float *, double *, Ctl *, int, int, int);
 
 }  // namespace pf

@@ -105,4 +105,27 @@ struct DevSym {
     const int *ld_ptr, *ld_node, *ld_d;     // flattened destinations per level
 };
 
+// The tail's triangular sweeps with its unknowns in LDS (TailSweeps on the
+// host): nt = t1 - t0 tail nodes, nfi forward and nbi backward entries on
+// tail nodes.
+struct DevTail {
+    int t0, t1, nfi, nbi;
+    const int *fx_ptr, *fx_k, *fx_q, *fi_ptr, *fi_k, *fi_q;
+    const int *bx_ptr, *bx_i, *bx_p, *bi_ptr, *bi_i, *bi_p;
+};
+
+// bytes of dynamic LDS of the tail sweeps: x of the tail nodes (2 doubles
+// each), the level bounds, the entry pointers and local indices, and the
+// entries' factor blocks (staged) or their slots (read from memory); the
+// backward sweep also stages the inverse pivots
+template <typename T>
+inline long long tail_sweep_lds(int nt, int nlev_tail, int nent, bool staged, bool backward) {
+    long long b = 16LL * nt;
+    if (staged) b += (long long)sizeof(T) * 4 * nent;
+    if (backward) b += (long long)sizeof(T) * 4 * nt;
+    b += 4LL * (nlev_tail + 1) + 4LL * (nt + 1) + 4LL * nent;
+    if (!staged) b += 4LL * nent;
+    return b;
+}
+
 }  // namespace pf

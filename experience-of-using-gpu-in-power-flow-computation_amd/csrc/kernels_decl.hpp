@@ -24,6 +24,12 @@ template <typename T> __global__ void k_fwd_level(DevSym Y, const T *blk, double
 template <typename T> __global__ void k_bwd_level(DevSym Y, const T *blk, double *x, Ctl *ctl, int l0, int count, int gate);
 template <typename T> __global__ void k_fwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate);
 template <typename T> __global__ void k_bwd_tail(DevSym Y, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate);
+template <typename T> __global__ void k_tail_fwd_ext(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int gate);
+template <typename T> __global__ void k_tail_bwd_ext(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int gate);
+template <typename T, bool STAGED>
+__global__ void k_fwd_tail_lds(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate);
+template <typename T, bool STAGED>
+__global__ void k_bwd_tail_lds(DevSym Y, DevTail D, const T *blk, double *x, Ctl *ctl, int lv0, int lv1, int gate);
 __global__ void k_copy(double *dst, const double *src, int n, Ctl *ctl, int gate);
 __global__ void k_axpy(double *x, const double *d, int n, Ctl *ctl, int gate);
 __global__ void k_fill(double *x, size_t n, double v);

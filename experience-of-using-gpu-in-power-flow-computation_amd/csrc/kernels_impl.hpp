@@ -74,6 +74,26 @@ template <typename T, bool WITH_U, int UNR>
 __device__ __forceinline__ void pairs_acc(const DevSym &Y, const T *blk, int q, int qe, int step, B4<T> &a,
                                           B4<T> &u) {
     const int L0 = Y.n, U0 = Y.n + Y.nnzL;
+    if constexpr (UNR == 1) {
+        // one pair at a time, its successor's indices loaded while its
+        // blocks are: one dependent load round per pair instead of two
+        if (q >= qe) return;
+        int pa = Y.fc_a[q], pb = Y.fc_b[q];
+        for (;;) {
+            const int qn = q + step;
+            int na = 0, nb = 0;
+            if (qn < qe) {
+                na = Y.fc_a[qn];
+                nb = Y.fc_b[qn];
+            }
+            mma2(a, load_b4(blk, L0 + pa), load_b4(blk, U0 + pb));
+            if (WITH_U) mma2(u, load_b4(blk, L0 + pb), load_b4(blk, U0 + pa));
+            if (qn >= qe) return;
+            q = qn;
+            pa = na;
+            pb = nb;
+        }
+    }
     for (; q + (UNR - 1) * step < qe; q += UNR * step) {
         int pa[UNR], pb[UNR];
 #pragma unroll
@@ -186,6 +206,114 @@ __device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *
         }
         __syncthreads();
         cur = nxt;
+    }
+}
+
+// Levels [lv0, lv1) of the chunked tail (Symbolic::ck_*): the update pairs
+// of each level cut into chunks of a few pairs, each inside one
+// destination, so that one round of loads covers a level whatever its
+// longest destination:
+//   A. lane groups of G sum one chunk each (pair q0 + lane + G*i, tree over
+//      the group) into LDS (s_pa, s_pu: one per chunk of the level);
+//   B. one thread per destination adds its chunks in order and updates the
+//      block; an updated pivot also goes to LDS (s_piv, one per node of the
+//      level);
+//   C. one thread per destination inverts the pivot into the inverse-pivot
+//      region or scales its column block (both L copies).
+// Three barriers per level; the sums are in a fixed order, whatever the
+// workgroup size.  What B and C need that does not depend on values (the
+// destination's indices, chunk range, current blocks, row-copy position) is
+// loaded before A, and C takes the pivot from LDS and the updated column
+// block from B's registers, so only A waits on global loads.
+template <typename T, int G, int UNR>
+__device__ void chunk_factor_levels(const DevSym &Y, T *blk, int lv0, int lv1, int *bad, B4<T> *s_pa,
+                                    B4<T> *s_pu, B4<T> *s_piv) {
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / G, lane = tid % G, ng = bd / G;
+    for (int l = lv0; l < lv1; l++) {
+        const int c0 = Y.ck_lvl[l - Y.tail_start], nc = Y.ck_lvl[l - Y.tail_start + 1] - c0;
+        const int e0 = Y.ld_ptr[l] - Y.te0, ne = Y.ld_ptr[l + 1] - Y.ld_ptr[l], s0 = Y.lvl_ptr[l];
+        // this thread's first destination, consumed after the chunk pass: its
+        // node, column position (-1: the pivot), chunk range, current blocks
+        // and row-copy position
+        int mj = -1, mp = -1, mcb = 0, mce = 0, mlrp = 0;
+        B4<T> mv0{0, 0, 0, 0}, mv1{0, 0, 0, 0};
+        auto fetch = [&](int E, int &j, int &p, int &cb, int &ce, int &lrp, B4<T> &v0,
+                         B4<T> &v1) __attribute__((always_inline)) {
+            j = Y.te_node[E];
+            p = Y.te_p[E];
+            cb = Y.te_ck[E];
+            ce = Y.te_ck[E + 1];
+            if (p < 0) {
+                v0 = load_b4(blk, j);
+            } else {
+                v0 = load_b4(blk, L0 + p);
+                v1 = load_b4(blk, U0 + p);
+                lrp = Y.lr_pos[p];
+            }
+        };
+        if (tid < ne) fetch(e0 + tid, mj, mp, mcb, mce, mlrp, mv0, mv1);
+        for (int c = grp; c < nc; c += ng) {
+            const int q0 = Y.ck_q0[c0 + c], q1f = Y.ck_q1[c0 + c], q1 = q1f & kChunkMask;
+            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+            if (q1f & kChunkDiag) pairs_acc<T, false, UNR>(Y, blk, q0 + lane, q1, G, a, u);
+            else pairs_acc<T, true, UNR>(Y, blk, q0 + lane, q1, G, a, u);
+            a = group_sum<T, G>(a);
+            u = group_sum<T, G>(u);
+            if (lane == 0) {
+                s_pa[c] = a;
+                s_pu[c] = u;
+            }
+        }
+        __syncthreads();
+        B4<T> keep{0, 0, 0, 0};
+        for (int e = tid; e < ne; e += bd) {
+            int j = mj, p = mp, cb = mcb, ce = mce, lrp = mlrp;
+            B4<T> v0 = mv0, v1 = mv1;
+            if (e != tid) fetch(e0 + e, j, p, cb, ce, lrp, v0, v1);
+            B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
+            for (int c = cb; c < ce; c++) {
+                const B4<T> x = s_pa[c - c0], y = s_pu[c - c0];
+                a = B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
+                u = B4<T>{u.a + y.a, u.b + y.b, u.c + y.c, u.d + y.d};
+            }
+            if (p < 0) {
+                const B4<T> P{v0.a - a.a, v0.b - a.b, v0.c - a.c, v0.d - a.d};
+                store_b4(blk, j, P);
+                s_piv[j - s0] = P;
+            } else {
+                const B4<T> Lb{v0.a - a.a, v0.b - a.b, v0.c - a.c, v0.d - a.d};
+                store_b4(blk, L0 + p, Lb);
+                store_b4(blk, U0 + p, B4<T>{v1.a - u.a, v1.b - u.b, v1.c - u.c, v1.d - u.d});
+                if (e == tid) keep = Lb;
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < ne; e += bd) {
+            int j = mj, p = mp, lrp = mlrp;
+            B4<T> Lb = keep;
+            if (e != tid) {
+                const int E = e0 + e;
+                j = Y.te_node[E];
+                p = Y.te_p[E];
+                lrp = p >= 0 ? Y.lr_pos[p] : 0;
+                Lb = p >= 0 ? load_b4(blk, L0 + p) : B4<T>{0, 0, 0, 0};
+            }
+            const B4<T> P = s_piv[j - s0];
+            T det = P.a * P.d - P.b * P.c;
+            const bool sing = !(det != (T)0) || !isfinite((double)det);
+            if (sing) det = (T)1;
+            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
+            if (p < 0) {
+                if (sing) atomicOr(bad, 1);
+                store_b4(blk, DI + j, Pi);
+            } else {
+                const B4<T> lb = mul(Lb, Pi);
+                store_b4(blk, L0 + p, lb);
+                store_b4(blk, LR0 + lrp, lb);
+            }
+        }
+        __syncthreads();
     }
 }
 

@@ -122,8 +122,27 @@ Symbolic analyze(const pf_network &net, const YPattern &yp, int tail_width, int 
 
 constexpr int kChunkPairs = 16;
 constexpr int kChunkDiag = 1 << 30, kChunkMask = kChunkDiag - 1;
-// Builds the chunk lists of the tail levels [tail_start, nlev_s).
-void chunk_tail(Symbolic &s);
+// Builds the chunk lists of the tail levels [tail_start, nlev_s), chunks of
+// at most chunk_pairs update pairs.
+void chunk_tail(Symbolic &s, int chunk_pairs);
+
+// The triangular sweeps of the tail nodes [t0, t1) (t0 = lvl_ptr[tail_start],
+// t1 = lvl_ptr[nlev_s]) with the tail's own unknowns held by one workgroup:
+// every forward row and backward column of a tail node is split into the
+// entries on nodes outside the tail, whose values are final before the sweep
+// starts (forward: nodes < t0; backward: the dense top, nodes >= t1), and the
+// entries on tail nodes, walked level by level.  Per tail node a = j - t0:
+//   forward  fx[fx_ptr[a], fx_ptr[a+1]): (node k < t0, row-ordered L slot q)
+//            fi[fi_ptr[a], fi_ptr[a+1]): (tail-local k - t0, q)
+//   backward bx[bx_ptr[a], bx_ptr[a+1]): (node i >= t1, L position p)
+//            bi[bi_ptr[a], bi_ptr[a+1]): (tail-local i - t0, p)
+// in the order of the row (ascending k) and of the column (ascending i).
+struct TailSweeps {
+    int t0 = 0, t1 = 0;
+    std::vector<int> fx_ptr, fx_k, fx_q, fi_ptr, fi_k, fi_q;
+    std::vector<int> bx_ptr, bx_i, bx_p, bi_ptr, bi_i, bi_p;
+};
+TailSweeps tail_sweeps(const Symbolic &s);
 
 // Splits the levels below s.tail_start into at most max_parts groups of
 // whole subtrees, balanced by factorisation work (longest processing time

@@ -60,6 +60,12 @@ struct pf_solver {
     DevNet dn{};
     DevState ds{};
     DevSym dy{};
+    // tail sweeps with x in LDS (blocked program): per precision [double,
+    // float] 0 = not used (the level walk through memory), 1 = entry slots
+    // in LDS, 2 = entry blocks staged in LDS; bytes of dynamic LDS per sweep
+    DevTail dtl{};
+    int tail_mode[2] = {0, 0};
+    int tail_lds_f[2] = {0, 0}, tail_lds_b[2] = {0, 0};
     double *blk64 = nullptr;
     float *blk32 = nullptr;
     double *dense = nullptr;
@@ -141,6 +147,7 @@ struct pf_solver {
     pf_status reset_ctl();
     pf_status enqueue_head(int gate);
     pf_status enqueue_factor_solve(int gate);
+    pf_status setup_tail_sweeps();
     template <typename T> pf_status enqueue_lu(T *blk, int gate);
     template <typename T> pf_status enqueue_solve(const T *blk, double *v, int gate);
     pf_status enqueue_step(int gate);
@@ -246,7 +253,7 @@ pf_status pf_solver::build() {
         HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
         partition_subtrees(sym, std::max(1, cus));
     }
-    chunk_tail(sym);
+    chunk_tail(sym, kChunkPairs);
     const bool chunked = sym.ck_max <= kMaxChunks && !sym.ck_q0.empty();
     dy.tail_start = sym.tail_start;
     dy.te0 = sym.ld_ptr[sym.tail_start];
@@ -259,6 +266,7 @@ pf_status pf_solver::build() {
         PFCHK(upload((int **)&dy.te_p, sym.te_p));
         PFCHK(upload((int **)&dy.te_ck, sym.te_ck));
     }
+    if (opt.lu_blocked && sym.tail_start < sym.nlev_s) PFCHK(setup_tail_sweeps());
     dy.n_parts = sym.n_parts;
     dy.part_levels = sym.tail_start;
     PFCHK(upload((int **)&dy.part_ptr, sym.part_ptr));
@@ -355,6 +363,62 @@ pf_status pf_solver::enqueue_head(int gate) {
     return PF_OK;
 }
 
+// The tail's sweeps with its unknowns in LDS, per precision as the LDS
+// holds them: the entries' factor blocks staged, else only their slots.
+pf_status pf_solver::setup_tail_sweeps() {
+    const TailSweeps ts = tail_sweeps(sym);
+    const int nt = ts.t1 - ts.t0, nlt = sym.nlev_s - sym.tail_start;
+    dtl.t0 = ts.t0;
+    dtl.t1 = ts.t1;
+    dtl.nfi = (int)ts.fi_k.size();
+    dtl.nbi = (int)ts.bi_i.size();
+    PFCHK(upload((int **)&dtl.fx_ptr, ts.fx_ptr));
+    PFCHK(upload((int **)&dtl.fx_k, ts.fx_k));
+    PFCHK(upload((int **)&dtl.fx_q, ts.fx_q));
+    PFCHK(upload((int **)&dtl.fi_ptr, ts.fi_ptr));
+    PFCHK(upload((int **)&dtl.fi_k, ts.fi_k));
+    PFCHK(upload((int **)&dtl.fi_q, ts.fi_q));
+    PFCHK(upload((int **)&dtl.bx_ptr, ts.bx_ptr));
+    PFCHK(upload((int **)&dtl.bx_i, ts.bx_i));
+    PFCHK(upload((int **)&dtl.bx_p, ts.bx_p));
+    PFCHK(upload((int **)&dtl.bi_ptr, ts.bi_ptr));
+    PFCHK(upload((int **)&dtl.bi_i, ts.bi_i));
+    PFCHK(upload((int **)&dtl.bi_p, ts.bi_p));
+    int max_lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    const long long cap = max_lds - 8192;   // room for the kernels' static LDS
+    auto pick = [&](int pi, long long f2, long long b2, long long f1, long long b1) {
+        if (f2 <= cap && b2 <= cap) {
+            tail_mode[pi] = 2, tail_lds_f[pi] = (int)f2, tail_lds_b[pi] = (int)b2;
+        } else if (f1 <= cap && b1 <= cap) {
+            tail_mode[pi] = 1, tail_lds_f[pi] = (int)f1, tail_lds_b[pi] = (int)b1;
+        }
+    };
+    pick(0, tail_sweep_lds<double>(nt, nlt, dtl.nfi, true, false), tail_sweep_lds<double>(nt, nlt, dtl.nbi, true, true),
+         tail_sweep_lds<double>(nt, nlt, dtl.nfi, false, false), tail_sweep_lds<double>(nt, nlt, dtl.nbi, false, true));
+    pick(1, tail_sweep_lds<float>(nt, nlt, dtl.nfi, true, false), tail_sweep_lds<float>(nt, nlt, dtl.nbi, true, true),
+         tail_sweep_lds<float>(nt, nlt, dtl.nfi, false, false), tail_sweep_lds<float>(nt, nlt, dtl.nbi, false, true));
+    auto allow = [&](const void *k, int bytes) -> pf_status {
+        HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        return PF_OK;
+    };
+    if (tail_mode[0] == 2) {
+        PFCHK(allow((const void *)k_fwd_tail_lds<double, true>, tail_lds_f[0]));
+        PFCHK(allow((const void *)k_bwd_tail_lds<double, true>, tail_lds_b[0]));
+    } else if (tail_mode[0] == 1) {
+        PFCHK(allow((const void *)k_fwd_tail_lds<double, false>, tail_lds_f[0]));
+        PFCHK(allow((const void *)k_bwd_tail_lds<double, false>, tail_lds_b[0]));
+    }
+    if (tail_mode[1] == 2) {
+        PFCHK(allow((const void *)k_fwd_tail_lds<float, true>, tail_lds_f[1]));
+        PFCHK(allow((const void *)k_bwd_tail_lds<float, true>, tail_lds_b[1]));
+    } else if (tail_mode[1] == 1) {
+        PFCHK(allow((const void *)k_fwd_tail_lds<float, false>, tail_lds_f[1]));
+        PFCHK(allow((const void *)k_bwd_tail_lds<float, false>, tail_lds_b[1]));
+    }
+    return PF_OK;
+}
+
 template <typename T>
 pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     // the threads past the slots clear the dense top block
@@ -391,10 +455,21 @@ pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
             k_fwd_level<T><<<(c + 15) / 16, 256, 0, stream>>>(dy, blk, v, ctl, sym.lvl_ptr[l], c, gate);
         }
     }
-    if (sym.tail_start < sym.nlev)
-        k_fwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
-    if (sym.tail_start < sym.nlev_s)
-        k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
+    const int pi = sizeof(T) == 4 ? 1 : 0, tm = sym.tail_start < sym.nlev_s ? tail_mode[pi] : 0;
+    const int lv0 = sym.tail_start, lv1 = sym.nlev_s, text = (dtl.t1 - dtl.t0 + 15) / 16;
+    if (tm) {
+        k_tail_fwd_ext<T><<<text, 256, 0, stream>>>(dy, dtl, blk, v, ctl, gate);
+        if (tm == 2) k_fwd_tail_lds<T, true><<<1, 1024, tail_lds_f[pi], stream>>>(dy, dtl, blk, v, ctl, lv0, lv1, gate);
+        else k_fwd_tail_lds<T, false><<<1, 1024, tail_lds_f[pi], stream>>>(dy, dtl, blk, v, ctl, lv0, lv1, gate);
+        if (sym.ch_m > 0) k_tail_bwd_ext<T><<<text, 256, 0, stream>>>(dy, dtl, blk, v, ctl, gate);
+        if (tm == 2) k_bwd_tail_lds<T, true><<<1, 1024, tail_lds_b[pi], stream>>>(dy, dtl, blk, v, ctl, lv0, lv1, gate);
+        else k_bwd_tail_lds<T, false><<<1, 1024, tail_lds_b[pi], stream>>>(dy, dtl, blk, v, ctl, lv0, lv1, gate);
+    } else {
+        if (sym.tail_start < sym.nlev)
+            k_fwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
+        if (sym.tail_start < sym.nlev_s)
+            k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
+    }
     if (sym.n_parts > 0) {
         k_bwd_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, v, ctl, gate);
     } else {

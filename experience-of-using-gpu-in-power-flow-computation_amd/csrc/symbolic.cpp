@@ -368,7 +368,7 @@ void partition_subtrees(Symbolic &s, int max_parts) {
     for (int j = 0; j < nw; j++) s.part_nodes[fill[(size_t)part[j] * T + s.level[j]]++] = j;
 }
 
-void chunk_tail(Symbolic &s) {
+void chunk_tail(Symbolic &s, int chunk_pairs) {
     s.ck_lvl.assign(1, 0);
     s.ck_q0.clear();
     s.ck_q1.clear();
@@ -383,15 +383,52 @@ void chunk_tail(Symbolic &s) {
             s.te_node.push_back(j);
             s.te_p.push_back(t == 0 ? -1 : s.colptr[j] + t - 1);
             s.te_ck.push_back((int)s.ck_q0.size());
-            for (int q = s.fc_ptr[d]; q < s.fc_ptr[d + 1]; q += kChunkPairs) {
+            for (int q = s.fc_ptr[d]; q < s.fc_ptr[d + 1]; q += chunk_pairs) {
                 s.ck_q0.push_back(q);
-                s.ck_q1.push_back(std::min(q + kChunkPairs, s.fc_ptr[d + 1]) | (t == 0 ? kChunkDiag : 0));
+                s.ck_q1.push_back(std::min(q + chunk_pairs, s.fc_ptr[d + 1]) | (t == 0 ? kChunkDiag : 0));
             }
         }
         s.ck_max = std::max(s.ck_max, (int)s.ck_q0.size() - before);
         s.ck_lvl.push_back((int)s.ck_q0.size());
     }
     s.te_ck.push_back((int)s.ck_q0.size());
+}
+
+TailSweeps tail_sweeps(const Symbolic &s) {
+    TailSweeps t;
+    t.t0 = s.lvl_ptr[s.tail_start];
+    t.t1 = s.lvl_ptr[s.nlev_s];
+    t.fx_ptr.assign(1, 0);
+    t.fi_ptr.assign(1, 0);
+    t.bx_ptr.assign(1, 0);
+    t.bi_ptr.assign(1, 0);
+    for (int j = t.t0; j < t.t1; j++) {
+        for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) {
+            const int k = s.rs_k[q];
+            if (k < t.t0) {
+                t.fx_k.push_back(k);
+                t.fx_q.push_back(q);
+            } else {
+                t.fi_k.push_back(k - t.t0);
+                t.fi_q.push_back(q);
+            }
+        }
+        for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) {
+            const int i = s.rowind[p];
+            if (i >= t.t1) {
+                t.bx_i.push_back(i);
+                t.bx_p.push_back(p);
+            } else {
+                t.bi_i.push_back(i - t.t0);
+                t.bi_p.push_back(p);
+            }
+        }
+        t.fx_ptr.push_back((int)t.fx_k.size());
+        t.fi_ptr.push_back((int)t.fi_k.size());
+        t.bx_ptr.push_back((int)t.bx_i.size());
+        t.bi_ptr.push_back((int)t.bi_i.size());
+    }
+    return t;
 }
 
 // ------------------------------------------------------------- host numeric

@@ -114,6 +114,22 @@ struct DevTail {
     const int *bx_ptr, *bx_i, *bx_p, *bi_ptr, *bi_i, *bi_p;
 };
 
+// The tail levels' factorisation with flat pair lists (FlatTail on the
+// host); E0 = ld_ptr[tail_start].
+struct DevFlat {
+    int E0, max_pairs, max_col, max_nodes;
+    const int *fl_ptr, *fl_split, *fl_a, *fl_b;
+    const int *de_node, *de_p, *de_ext, *de_lo, *de_hi;
+};
+
+// bytes of dynamic LDS of the flat tail factorisation: the pair products of
+// one level (the second product only for column destinations) and the
+// level's updated pivots
+template <typename T>
+inline long long flat_tail_lds(int max_pairs, int max_col, int max_nodes) {
+    return (long long)sizeof(T) * 4 * ((long long)max_pairs + max_col + max_nodes);
+}
+
 // bytes of dynamic LDS of the tail sweeps: x of the tail nodes (2 doubles
 // each), the level bounds, the entry pointers and local indices, and the
 // entries' factor blocks (staged) or their slots (read from memory); the

@@ -399,6 +399,11 @@ __device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B
 // (requires ceil(m / TR) * ceil(m / TC) <= blockDim).  Row k and column k go
 // through LDS (lds: 4 ch_m blocks, double buffered: one barrier per step); a
 // thread reads TR + TC blocks of them per step, not two per block it owns.
+// In FP32 every block of a tile takes the general update
+// S_ij -= S_ik (P S_kj) without a test; the few threads holding row k or
+// column k then overwrite those blocks with P S_kj, -S_ik P and P, so the
+// update loop carries no per-block branches (blocks past m are padding,
+// updated but never stored).  Both forms give the same values.
 // The caller orders it after chain_schur and synchronises after.
 template <typename T, int TR, int TC>
 __device__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
@@ -413,8 +418,12 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
 #pragma unroll
         for (int c = 0; c < TC; c++)
             reg[a][c] = (i0 + a < m && j0 + c < m) ? load_b4(blk, SB + (i0 + a) * m + j0 + c) : B4<T>{0, 0, 0, 0};
+    // LDS index of tile column c / row a, clamped into [0, m)
+    auto jc = [&](int c) { return min(j0 + c, m - 1); };
+    auto ic = [&](int a) { return min(i0 + a, m - 1); };
     for (int k = 0; k < m; k++) {
         B4<T> *s_row = lds + (k & 1) * 2 * m, *s_col = s_row + m;
+        const int ar = k - i0, ac = k - j0;   // pivot row / column inside this tile when in [0, TR) / [0, TC)
         if (k >= i0 && k < i0 + TR) {   // this thread holds part of row k
 #pragma unroll
             for (int a = 0; a < TR; a++)
@@ -435,48 +444,57 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
         }
         __syncthreads();
         const B4<T> P = pivot_inverse(s_row[k], bad);
+        if (!own) continue;
         if constexpr (sizeof(T) == 4) {   // registers to spare: read row and column once
             B4<T> pr[TC], ci[TR];
 #pragma unroll
-            for (int c = 0; c < TC; c++) pr[c] = (j0 + c < m && j0 + c != k) ? mul(P, s_row[j0 + c]) : P;
+            for (int c = 0; c < TC; c++) pr[c] = mul(P, s_row[jc(c)]);
 #pragma unroll
-            for (int a = 0; a < TR; a++) ci[a] = s_col[i0 + a < m ? i0 + a : 0];
+            for (int a = 0; a < TR; a++) ci[a] = s_col[ic(a)];
 #pragma unroll
-            for (int a = 0; a < TR; a++) {
-                const int i = i0 + a;
+            for (int a = 0; a < TR; a++)
 #pragma unroll
-                for (int c = 0; c < TC; c++) {
-                    const int j = j0 + c;
-                    if (i >= m || j >= m) continue;
+                for (int c = 0; c < TC; c++) mms(reg[a][c], ci[a], pr[c]);
+            if ((unsigned)ar < (unsigned)TR) {   // row k: P S_kj
+#pragma unroll
+                for (int a = 0; a < TR; a++)
+                    if (a == ar) {
+#pragma unroll
+                        for (int c = 0; c < TC; c++) reg[a][c] = pr[c];
+                    }
+            }
+            if ((unsigned)ac < (unsigned)TC) {   // column k: -S_ik P, and P on the pivot
+#pragma unroll
+                for (int c = 0; c < TC; c++)
+                    if (c == ac) {
+#pragma unroll
+                        for (int a = 0; a < TR; a++) {
+                            const B4<T> v = mul(ci[a], P);
+                            reg[a][c] = (a == ar) ? P : B4<T>{-v.a, -v.b, -v.c, -v.d};
+                        }
+                    }
+            }
+        } else {
+            // column by column: one P S_kj live at a time, S_ik re-read from
+            // LDS; the register budget leaves no room for the unconditional
+            // form, so the pivot row and column are tested per block
+#pragma unroll
+            for (int c = 0; c < TC; c++) {
+                const int j = j0 + c;
+                if (j >= m) continue;
+                const B4<T> prc = (j != k) ? mul(P, s_row[j]) : P;
+#pragma unroll
+                for (int a = 0; a < TR; a++) {
+                    const int i = i0 + a;
+                    if (i >= m) continue;
                     if (i == k) {
-                        reg[a][c] = pr[c];
+                        reg[a][c] = prc;
                     } else if (j == k) {
-                        const B4<T> v = mul(ci[a], P);
+                        const B4<T> v = mul(s_col[i], P);
                         reg[a][c] = B4<T>{-v.a, -v.b, -v.c, -v.d};
                     } else {
-                        mms(reg[a][c], ci[a], pr[c]);
+                        mms(reg[a][c], s_col[i], prc);
                     }
-                }
-            }
-            continue;
-        }
-        // column by column: one P S_kj live at a time, S_ik re-read from LDS
-#pragma unroll
-        for (int c = 0; c < TC; c++) {
-            const int j = j0 + c;
-            if (j >= m) continue;
-            const B4<T> pr = (j != k) ? mul(P, s_row[j]) : P;
-#pragma unroll
-            for (int a = 0; a < TR; a++) {
-                const int i = i0 + a;
-                if (i >= m) continue;
-                if (i == k) {
-                    reg[a][c] = pr;
-                } else if (j == k) {
-                    const B4<T> v = mul(s_col[i], P);
-                    reg[a][c] = B4<T>{-v.a, -v.b, -v.c, -v.d};
-                } else {
-                    mms(reg[a][c], s_col[i], pr);
                 }
             }
         }

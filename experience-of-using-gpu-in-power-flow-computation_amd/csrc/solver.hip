@@ -64,6 +64,11 @@ struct pf_solver {
     // float] 0 = not used (the level walk through memory), 1 = entry slots
     // in LDS, 2 = entry blocks staged in LDS; bytes of dynamic LDS per sweep
     DevTail dtl{};
+    // flat tail factorisation (blocked program): per precision [double,
+    // float] its dynamic LDS bytes, 0 = not used (the chunked walk)
+    DevFlat dfl{};
+    int flat_ne = 0;
+    int flat_lds[2] = {0, 0};
     int tail_mode[2] = {0, 0};
     int tail_lds_f[2] = {0, 0}, tail_lds_b[2] = {0, 0};
     double *blk64 = nullptr;
@@ -148,6 +153,7 @@ struct pf_solver {
     pf_status enqueue_head(int gate);
     pf_status enqueue_factor_solve(int gate);
     pf_status setup_tail_sweeps();
+    pf_status setup_flat_tail();
     template <typename T> pf_status enqueue_lu(T *blk, int gate);
     template <typename T> pf_status enqueue_solve(const T *blk, double *v, int gate);
     pf_status enqueue_step(int gate);
@@ -266,7 +272,10 @@ pf_status pf_solver::build() {
         PFCHK(upload((int **)&dy.te_p, sym.te_p));
         PFCHK(upload((int **)&dy.te_ck, sym.te_ck));
     }
-    if (opt.lu_blocked && sym.tail_start < sym.nlev_s) PFCHK(setup_tail_sweeps());
+    if (opt.lu_blocked && sym.tail_start < sym.nlev_s) {
+        PFCHK(setup_tail_sweeps());
+        PFCHK(setup_flat_tail());
+    }
     dy.n_parts = sym.n_parts;
     dy.part_levels = sym.tail_start;
     PFCHK(upload((int **)&dy.part_ptr, sym.part_ptr));
@@ -419,6 +428,42 @@ pf_status pf_solver::setup_tail_sweeps() {
     return PF_OK;
 }
 
+// The tail levels' factorisation with flat pair lists, per precision as
+// the LDS holds one level's products.
+pf_status pf_solver::setup_flat_tail() {
+    const FlatTail ft = flat_tail(sym);
+    flat_ne = (int)ft.de_node.size();
+    dfl.E0 = sym.ld_ptr[sym.tail_start];
+    dfl.max_pairs = ft.max_pairs;
+    dfl.max_col = ft.max_col;
+    dfl.max_nodes = ft.max_nodes;
+    PFCHK(upload((int **)&dfl.fl_ptr, ft.fl_ptr));
+    PFCHK(upload((int **)&dfl.fl_split, ft.fl_split));
+    PFCHK(upload((int **)&dfl.fl_a, ft.fl_a));
+    PFCHK(upload((int **)&dfl.fl_b, ft.fl_b));
+    PFCHK(upload((int **)&dfl.de_node, ft.de_node));
+    PFCHK(upload((int **)&dfl.de_p, ft.de_p));
+    PFCHK(upload((int **)&dfl.de_ext, ft.de_ext));
+    PFCHK(upload((int **)&dfl.de_lo, ft.de_lo));
+    PFCHK(upload((int **)&dfl.de_hi, ft.de_hi));
+    int max_lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    const long long cap = max_lds - 8192;
+    const long long b64 = flat_tail_lds<double>(ft.max_pairs, ft.max_col, ft.max_nodes);
+    const long long b32 = flat_tail_lds<float>(ft.max_pairs, ft.max_col, ft.max_nodes);
+    if (b64 <= cap) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_factor_tail_flat<double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)b64));
+        flat_lds[0] = (int)std::max<long long>(b64, 16);
+    }
+    if (b32 <= cap) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_factor_tail_flat<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)b32));
+        flat_lds[1] = (int)std::max<long long>(b32, 16);
+    }
+    return PF_OK;
+}
+
 template <typename T>
 pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     // the threads past the slots clear the dense top block
@@ -434,8 +479,13 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
             k_factor_level<T><<<(c + 3) / 4, 256, 0, stream>>>(dy, blk, ctl, sym.lvl_ptr[l], c, gate);
         }
     }
-    if (sym.tail_start < sym.nlev_s)   // narrow levels below the chain
+    const int fl = sym.tail_start < sym.nlev_s ? flat_lds[sizeof(T) == 4 ? 1 : 0] : 0;
+    if (fl) {   // narrow levels below the chain: pairs from below the tail across the GPU first
+        k_tail_factor_ext<T><<<(flat_ne + 31) / 32, 256, 0, stream>>>(dy, dfl, blk, ctl, flat_ne, gate);
+        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.tail_start, sym.nlev_s, gate);
+    } else if (sym.tail_start < sym.nlev_s) {
         k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev_s, gate);
+    }
     if (sym.ch_m > 0) {   // the dense chain: Schur complement across the GPU, then its inverse
         const int ne = sym.fd_ptr[sym.n] - sym.fd_ptr[sym.ch_c0];
         k_chain_schur<T><<<(ne + 15) / 16, 256, 0, stream>>>(dy, blk, ctl, gate);

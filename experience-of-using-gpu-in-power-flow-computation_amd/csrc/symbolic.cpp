@@ -431,6 +431,49 @@ TailSweeps tail_sweeps(const Symbolic &s) {
     return t;
 }
 
+FlatTail flat_tail(const Symbolic &s) {
+    FlatTail f;
+    const int t0 = s.lvl_ptr[s.tail_start], E0 = s.ld_ptr[s.tail_start];
+    std::vector<int> pcol(s.nnzL);
+    for (int k = 0; k < s.n; k++)
+        for (int p = s.colptr[k]; p < s.colptr[k + 1]; p++) pcol[p] = k;
+    const int ne = s.ld_ptr[s.nlev_s] - E0;
+    f.de_node.resize(ne);
+    f.de_p.resize(ne);
+    f.de_ext.resize(ne);
+    f.de_lo.resize(ne);
+    f.de_hi.resize(ne);
+    f.fl_ptr.assign(1, 0);
+    for (int l = s.tail_start; l < s.nlev_s; l++) {
+        const int base = f.fl_ptr.back();
+        for (int pass = 0; pass < 2; pass++) {   // the pivots' pairs, then the column destinations'
+            if (pass == 1) f.fl_split.push_back((int)f.fl_a.size() - base);
+            for (int e = s.ld_ptr[l]; e < s.ld_ptr[l + 1]; e++) {
+                const int j = s.ld_node[e], d = s.ld_d[e], t = d - s.fd_ptr[j];
+                if ((t == 0) != (pass == 0)) continue;
+                int q = s.fc_ptr[d];
+                while (q < s.fc_ptr[d + 1] && pcol[s.fc_b[q]] < t0) q++;   // pairs ascend in k
+                const int E = e - E0;
+                f.de_node[E] = j;
+                f.de_p[E] = t == 0 ? -1 : s.colptr[j] + t - 1;
+                f.de_ext[E] = q;
+                f.de_lo[E] = (int)f.fl_a.size() - base;
+                for (; q < s.fc_ptr[d + 1]; q++) {
+                    f.fl_a.push_back(s.fc_a[q]);
+                    f.fl_b.push_back(s.fc_b[q]);
+                }
+                f.de_hi[E] = (int)f.fl_a.size() - base;
+            }
+        }
+        const int np = (int)f.fl_a.size() - base;
+        f.max_pairs = std::max(f.max_pairs, np);
+        f.max_col = std::max(f.max_col, np - f.fl_split.back());
+        f.max_nodes = std::max(f.max_nodes, s.lvl_ptr[l + 1] - s.lvl_ptr[l]);
+        f.fl_ptr.push_back((int)f.fl_a.size());
+    }
+    return f;
+}
+
 // ------------------------------------------------------------- host numeric
 
 namespace {

@@ -40,6 +40,7 @@ struct BatchTopo {
     const double *V0flat, *th0flat;     // flat start of this topology
     int nslot;                          // Jacobian slots
     int nblk;                           // factor blocks: slots, row-ordered L, chain inverse
+    DevFlat F;                          // flat pair program of the narrow top (F.fl_ptr null: none)
 };
 
 struct BatchJob {
@@ -408,6 +409,151 @@ __device__ __forceinline__ void wg_solve(const DevSym &Y, const T *blk, double *
     }
 }
 
+// Sum of the products [lo, hi) of an LDS array in index order (four
+// loads in flight before their additions, which keep the order).
+__device__ __forceinline__ double2 prod_sum_in_order(const double2 *v, int lo, int hi) {
+    double a0 = 0, a1 = 0;
+    int q = lo;
+    for (; q + 4 <= hi; q += 4) {
+        const double2 p0 = v[q], p1 = v[q + 1], p2 = v[q + 2], p3 = v[q + 3];
+        a0 += p0.x; a1 += p0.y;
+        a0 += p1.x; a1 += p1.y;
+        a0 += p2.x; a1 += p2.y;
+        a0 += p3.x; a1 += p3.y;
+    }
+    for (; q < hi; q++) {
+        a0 += v[q].x;
+        a1 += v[q].y;
+    }
+    return make_double2(a0, a1);
+}
+
+// Forward and backward sweeps with one entry per lane: on every level each
+// lane forms one product L_jk x_k (backward U_ji x_i) into LDS (prod, page
+// of them at a time) and then one thread per node subtracts its products in
+// entry order (backward: and applies the inverse pivot after the level's
+// last page).  The entries of a level are contiguous (rs_ptr / colptr in
+// node order, nodes numbered level by level), so the entry a lane takes
+// first on the next level, its index and factor block (and the inverse
+// pivot of its first node), are loaded a level ahead: a level that fits one
+// page and one entry per lane waits on no global load.
+template <typename T>
+__device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, double *x, double *s_r, double2 *prod,
+                                              int page) {
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    const int LR0 = Y.n + 2 * Y.nnzL, U0 = Y.n + Y.nnzL, nl = Y.nlev_s;
+    int pk = -1;
+    B4<T> pB{0, 0, 0, 0};
+    auto fetch_f = [&](int l) __attribute__((always_inline)) {
+        const int e = Y.rs_ptr[Y.lvl_ptr[l]] + tid;
+        pk = -1;
+        if (e < Y.rs_ptr[Y.lvl_ptr[l + 1]]) {
+            pk = Y.rs_k[e];
+            pB = load_b4(blk, LR0 + e);
+        }
+    };
+    if (nl > 0) fetch_f(0);
+    for (int l = 0; l < nl; l++) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        const int e_lo = Y.rs_ptr[s], e_hi = Y.rs_ptr[s + c];
+        if (e_lo == e_hi && l + 1 < nl) fetch_f(l + 1);
+        for (int pg = e_lo; pg < e_hi; pg += page) {
+            const int pe = min(pg + page, e_hi);
+            for (int e = pg + tid; e < pe; e += bd) {
+                int k;
+                B4<T> L;
+                if (e == e_lo + tid) {
+                    k = pk;
+                    L = pB;
+                } else {
+                    k = Y.rs_k[e];
+                    L = load_b4(blk, LR0 + e);
+                }
+                const double x0 = x[2 * k], x1 = x[2 * k + 1];
+                prod[e - pg] = make_double2((double)L.a * x0 + (double)L.b * x1, (double)L.c * x0 + (double)L.d * x1);
+            }
+            if (pg == e_lo && l + 1 < nl) fetch_f(l + 1);
+            __syncthreads();
+            for (int i = tid; i < c; i += bd) {
+                const int j = s + i, lo = max(Y.rs_ptr[j], pg), hi = min(Y.rs_ptr[j + 1], pe);
+                if (lo < hi) {
+                    const double2 a = prod_sum_in_order(prod, lo - pg, hi - pg);
+                    x[2 * j] -= a.x;
+                    x[2 * j + 1] -= a.y;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (Y.ch_m > 0) chain_solve<T, GL>(Y, blk, x, s_r);
+    int pi = -1;
+    B4<T> pD{0, 0, 0, 0};
+    auto fetch_b = [&](int l) __attribute__((always_inline)) {
+        const int s = Y.lvl_ptr[l], e = Y.colptr[s] + tid;
+        pi = -1;
+        if (e < Y.colptr[Y.lvl_ptr[l + 1]]) {
+            pi = Y.rowind[e];
+            pB = load_b4(blk, U0 + e);
+        }
+        if (tid < Y.lvl_ptr[l + 1] - s) pD = load_b4(blk, Y.di0 + s + tid);
+    };
+    if (nl > 0) fetch_b(nl - 1);
+    for (int l = nl - 1; l >= 0; l--) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        const int e_lo = Y.colptr[s], e_hi = Y.colptr[s + c];
+        const B4<T> myD = pD;   // inverse pivot of node s + tid
+        if (e_lo == e_hi) {     // no entries: only the inverse pivots
+            if (l > 0) fetch_b(l - 1);
+            for (int i = tid; i < c; i += bd) {
+                const int j = s + i;
+                const B4<T> D = (i == tid) ? myD : load_b4(blk, Y.di0 + j);
+                const double y0 = x[2 * j], y1 = x[2 * j + 1];
+                x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
+                x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
+            }
+            __syncthreads();
+            continue;
+        }
+        for (int pg = e_lo; pg < e_hi; pg += page) {
+            const int pe = min(pg + page, e_hi);
+            for (int e = pg + tid; e < pe; e += bd) {
+                int r;
+                B4<T> U;
+                if (e == e_lo + tid) {
+                    r = pi;
+                    U = pB;
+                } else {
+                    r = Y.rowind[e];
+                    U = load_b4(blk, U0 + e);
+                }
+                const double x0 = x[2 * r], x1 = x[2 * r + 1];
+                prod[e - pg] = make_double2((double)U.a * x0 + (double)U.b * x1, (double)U.c * x0 + (double)U.d * x1);
+            }
+            if (pg == e_lo && l > 0) fetch_b(l - 1);
+            __syncthreads();
+            const bool last = pe == e_hi;
+            for (int i = tid; i < c; i += bd) {
+                const int j = s + i, lo = max(Y.colptr[j], pg), hi = min(Y.colptr[j + 1], pe);
+                double y0 = x[2 * j], y1 = x[2 * j + 1];
+                if (lo < hi) {
+                    const double2 a = prod_sum_in_order(prod, lo - pg, hi - pg);
+                    y0 -= a.x;
+                    y1 -= a.y;
+                }
+                if (last) {
+                    const B4<T> D = (i == tid) ? myD : load_b4(blk, Y.di0 + j);
+                    x[2 * j] = (double)D.a * y0 + (double)D.b * y1;
+                    x[2 * j + 1] = (double)D.c * y0 + (double)D.d * y1;
+                } else if (lo < hi) {
+                    x[2 * j] = y0;
+                    x[2 * j + 1] = y1;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // I = Y E and S = E conj(I): 4 lanes per row
 template <typename T>
 __device__ void wg_pq(const DevNet &N, Work<T> &w) {
@@ -629,7 +775,6 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
             }
             store_b4(w.blk, s, out);
         }
-        for (int i = tid; i < 2 * Y.n; i += blockDim.x) w.x[i] = w.g[i];
         if (tid == 0) s_maxDx_bits = 0;
         __syncthreads();
         // factor, level by level
@@ -638,6 +783,11 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
         {
             const int grp = tid / GL, lane = tid % GL, ng = (int)blockDim.x / GL;
             for (int l = 0; l < Y.nlev_s;) {
+                if (s_ck && tp.F.fl_ptr && l == tp.F.lv0 && ((size_t)w.x & 15) == 0) {   // flat pair lists
+                    T *pa = reinterpret_cast<T *>(w.x);   // x and r: free until the solve
+                    flat_factor_levels<T, 1, 4>(Y, tp.F, w.blk, Y.nlev_s, &s_sing, pa, pa + 4 * (size_t)tp.F.max_pairs);
+                    break;
+                }
                 if (s_ck && Y.ck_lvl && l >= Y.tail_start) {   // the chunked narrow top
                     chunk_factor_levels<T, kBatchChunkLanes, 1>(Y, w.blk, l, Y.nlev_s, &s_sing, s_ck,
                                                                   s_ck + kBatchMaxChunks, s_chain);
@@ -661,11 +811,17 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
             }
         }
         if (s_sing) break;
-        wg_solve(Y, (const T *)w.blk, w.x, (double *)s_chain);
+        for (int i = tid; i < 2 * Y.n; i += blockDim.x) w.x[i] = w.g[i];
+        __syncthreads();
+        // the flat sweeps take their products in s_ck (free after the factorisation)
+        const int page = (int)(sizeof(B4<T>) * 2 * kBatchMaxChunks / sizeof(double2));
+        if (s_ck) wg_solve_flat(Y, (const T *)w.blk, w.x, (double *)s_chain, reinterpret_cast<double2 *>(s_ck), page);
+        else wg_solve(Y, (const T *)w.blk, w.x, (double *)s_chain);
         for (int rs = 0; rs < J.refine; rs++) {
             wg_residual(N, w);
             __syncthreads();
-            wg_solve(Y, (const T *)w.blk, w.r, (double *)s_chain);
+            if (s_ck) wg_solve_flat(Y, (const T *)w.blk, w.r, (double *)s_chain, reinterpret_cast<double2 *>(s_ck), page);
+            else wg_solve(Y, (const T *)w.blk, w.r, (double *)s_chain);
             for (int i = tid; i < 2 * Y.n; i += blockDim.x) w.x[i] += w.r[i];
             __syncthreads();
         }
@@ -1094,6 +1250,36 @@ pf_status pf_batch::add_net(const pf_network &src) {
         BCHK(upload((int **)&Y.te_node, sym.te_node));
         BCHK(upload((int **)&Y.te_p, sym.te_p));
         BCHK(upload((int **)&Y.te_ck, sym.te_ck));
+    }
+    // the factorisation as flat pair lists, its products in the step vectors
+    // x and r (4 n doubles, free during the factorisation): from the first
+    // level when one level's products fit there, else from the narrow top
+    b->tp.F = DevFlat{};
+    if (opt.lu_blocked && sym.tail_start < sym.nlev_s) {
+        const bool mx = opt.precision != PF_FP64;
+        auto need = [&](const FlatTail &f) {
+            return mx ? flat_tail_lds<float>(f.max_pairs, f.max_col) : flat_tail_lds<double>(f.max_pairs, f.max_col);
+        };
+        FlatTail ft = flat_tail(sym, 0, false);
+        if (need(ft) > 32LL * sym.n) ft = flat_tail(sym, sym.tail_start, false);
+        if (need(ft) <= 32LL * sym.n) {
+            DevFlat &F = b->tp.F;
+            F.lv0 = ft.lv0;
+            F.E0 = sym.ld_ptr[ft.lv0];
+            F.max_pairs = ft.max_pairs;
+            F.max_col = ft.max_col;
+            BCHK(upload((int **)&F.fl_ptr, ft.fl_ptr));
+            BCHK(upload((int **)&F.fl_split, ft.fl_split));
+            BCHK(upload((int **)&F.fl_a, ft.fl_a));
+            BCHK(upload((int **)&F.fl_b, ft.fl_b));
+            BCHK(upload((int **)&F.de_node, ft.de_node));
+            BCHK(upload((int **)&F.de_p, ft.de_p));
+            BCHK(upload((int **)&F.de_ext, ft.de_ext));
+            BCHK(upload((int **)&F.de_lo, ft.de_lo));
+            BCHK(upload((int **)&F.de_hi, ft.de_hi));
+            BCHK(upload((int **)&F.de_dlo, ft.de_dlo));
+            BCHK(upload((int **)&F.de_dhi, ft.de_dhi));
+        }
     }
     BCHK(upload((double **)&b->tp.V0flat, net.V));
     BCHK(upload((double **)&b->tp.th0flat, net.theta));

@@ -372,133 +372,15 @@ __global__ void __launch_bounds__(256) k_tail_factor_ext(DevSym Y, DevFlat F, T 
     }
 }
 
-// Flat tail factorisation, second part: levels [lv0, lv1) (lv0 = the first
-// tail level) in one workgroup, three barriers per level:
-//   A. one update pair per lane (strided), its block product(s) into LDS;
-//      the lane's first pair indices of the next level are loaded meanwhile
-//   B. one thread per destination adds its products in pair order and
-//      updates the block; an updated pivot also goes to LDS;
-//   C. one thread per destination inverts the pivot into the inverse-pivot
-//      region or scales its column block (both L copies).
-// Only A waits on global loads, one round of them: the pair indices are in
-// registers from the level before.
+// Flat tail factorisation, second part: the tail levels in one workgroup
+// (flat_factor_levels), the level's products in dynamic LDS.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int lv0, int lv1,
-                                                           int gate) {
+__global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int lv1, int gate) {
     if (gate && !ctl->step) return;
     extern __shared__ __align__(16) char smem[];
-    T *s_pa = reinterpret_cast<T *>(smem);               // B4 blocks, max_pairs
-    T *s_pu = s_pa + 4 * (size_t)F.max_pairs;            // max_col
-    T *s_piv = s_pu + 4 * (size_t)F.max_col;             // max_nodes
-    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
-    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
-    int *bad = &ctl->singular;
-    auto pair_at = [&](int t, int &qa, int &qb) {       // this thread's first pair of tail level t
-        qa = qb = 0;
-        const int b = F.fl_ptr[t];
-        if (tid < F.fl_ptr[t + 1] - b) {
-            qa = F.fl_a[b + tid];
-            qb = F.fl_b[b + tid];
-        }
-    };
-    int pa = 0, pb = 0;
-    if (lv0 < lv1) pair_at(0, pa, pb);
-    for (int l = lv0; l < lv1; l++) {
-        const int t = l - lv0, base = F.fl_ptr[t], np = F.fl_ptr[t + 1] - base, ns = F.fl_split[t];
-        const int e0 = Y.ld_ptr[l] - F.E0, ne = Y.ld_ptr[l + 1] - Y.ld_ptr[l], s0 = Y.lvl_ptr[l];
-        // this thread's first destination (consumed in B and C): node,
-        // column position, product range, current blocks, row-copy position
-        int mj = 0, mp = -1, mlo = 0, mhi = 0, mlrp = 0;
-        B4<T> mv0{0, 0, 0, 0}, mv1{0, 0, 0, 0};
-        if (tid < ne) {
-            const int E = e0 + tid;
-            mj = F.de_node[E];
-            mp = F.de_p[E];
-            mlo = F.de_lo[E];
-            mhi = F.de_hi[E];
-            if (mp < 0) {
-                mv0 = load_b4(blk, mj);
-            } else {
-                mv0 = load_b4(blk, L0 + mp);
-                mv1 = load_b4(blk, U0 + mp);
-                mlrp = Y.lr_pos[mp];
-            }
-        }
-        int na = 0, nb = 0;
-        if (l + 1 < lv1) pair_at(t + 1, na, nb);
-        for (int e = tid; e < np; e += bd) {
-            const int qa = (e == tid) ? pa : F.fl_a[base + e], qb = (e == tid) ? pb : F.fl_b[base + e];
-            store_b4(s_pa, e, mul(load_b4(blk, L0 + qa), load_b4(blk, U0 + qb)));
-            if (e >= ns) store_b4(s_pu, e - ns, mul(load_b4(blk, L0 + qb), load_b4(blk, U0 + qa)));
-        }
-        pa = na;
-        pb = nb;
-        __syncthreads();
-        B4<T> keep{0, 0, 0, 0};
-        for (int e = tid; e < ne; e += bd) {
-            int j = mj, p = mp, lo = mlo, hi = mhi;
-            B4<T> v0 = mv0, v1 = mv1;
-            if (e != tid) {
-                const int E = e0 + e;
-                j = F.de_node[E];
-                p = F.de_p[E];
-                lo = F.de_lo[E];
-                hi = F.de_hi[E];
-                if (p < 0) {
-                    v0 = load_b4(blk, j);
-                } else {
-                    v0 = load_b4(blk, L0 + p);
-                    v1 = load_b4(blk, U0 + p);
-                }
-            }
-            B4<T> a{0, 0, 0, 0};
-            for (int q = lo; q < hi; q++) {
-                const B4<T> x = load_b4(s_pa, q);
-                a = B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
-            }
-            if (p < 0) {
-                const B4<T> P{v0.a - a.a, v0.b - a.b, v0.c - a.c, v0.d - a.d};
-                store_b4(blk, j, P);
-                store_b4(s_piv, j - s0, P);
-            } else {
-                B4<T> u{0, 0, 0, 0};
-                for (int q = lo; q < hi; q++) {
-                    const B4<T> y = load_b4(s_pu, q - ns);
-                    u = B4<T>{u.a + y.a, u.b + y.b, u.c + y.c, u.d + y.d};
-                }
-                const B4<T> Lb{v0.a - a.a, v0.b - a.b, v0.c - a.c, v0.d - a.d};
-                store_b4(blk, L0 + p, Lb);
-                store_b4(blk, U0 + p, B4<T>{v1.a - u.a, v1.b - u.b, v1.c - u.c, v1.d - u.d});
-                if (e == tid) keep = Lb;
-            }
-        }
-        __syncthreads();
-        for (int e = tid; e < ne; e += bd) {
-            int j = mj, p = mp, lrp = mlrp;
-            B4<T> Lb = keep;
-            if (e != tid) {
-                const int E = e0 + e;
-                j = F.de_node[E];
-                p = F.de_p[E];
-                lrp = p >= 0 ? Y.lr_pos[p] : 0;
-                Lb = p >= 0 ? load_b4(blk, L0 + p) : B4<T>{0, 0, 0, 0};
-            }
-            const B4<T> P = load_b4(s_piv, j - s0);
-            T det = P.a * P.d - P.b * P.c;
-            const bool sing = !(det != (T)0) || !isfinite((double)det);
-            if (sing) det = (T)1;
-            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
-            if (p < 0) {
-                if (sing) atomicOr(bad, 1);
-                store_b4(blk, DI + j, Pi);
-            } else {
-                const B4<T> lb = mul(Lb, Pi);
-                store_b4(blk, L0 + p, lb);
-                store_b4(blk, LR0 + lrp, lb);
-            }
-        }
-        __syncthreads();
-    }
+    T *s_pa = reinterpret_cast<T *>(smem);
+    flat_factor_levels<T, sizeof(T) == 4 ? 3 : 2, sizeof(T) == 4 ? 8 : 4>(Y, F, blk, lv1, &ctl->singular, s_pa,
+                                                                            s_pa + 4 * (size_t)F.max_pairs);
 }
 
 // Schur complement of the trailing dense chain: every destination is
@@ -1157,8 +1039,8 @@ template __global__ void k_factor_tail<double>(DevSym, double *, Ctl *, int, int
 template __global__ void k_factor_tail<float>(DevSym, float *, Ctl *, int, int, int);
 template __global__ void k_tail_factor_ext<double>(DevSym, DevFlat, double *, Ctl *, int, int);
 template __global__ void k_tail_factor_ext<float>(DevSym, DevFlat, float *, Ctl *, int, int);
-template __global__ void k_factor_tail_flat<double>(DevSym, DevFlat, double *, Ctl *, int, int, int);
-template __global__ void k_factor_tail_flat<float>(DevSym, DevFlat, float *, Ctl *, int, int, int);
+template __global__ void k_factor_tail_flat<double>(DevSym, DevFlat, double *, Ctl *, int, int);
+template __global__ void k_factor_tail_flat<float>(DevSym, DevFlat, float *, Ctl *, int, int);
 template __global__ void k_fwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);
 template __global__ void k_fwd_level<float>(DevSym, const float *, double *, Ctl *, int, int, int);
 template __global__ void k_bwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);

@@ -114,20 +114,19 @@ struct DevTail {
     const int *bx_ptr, *bx_i, *bx_p, *bi_ptr, *bi_i, *bi_p;
 };
 
-// The tail levels' factorisation with flat pair lists (FlatTail on the
-// host); E0 = ld_ptr[tail_start].
+// Levels [lv0, nlev_s) factored as flat pair lists (FlatTail on the host);
+// E0 = ld_ptr[lv0].
 struct DevFlat {
-    int E0, max_pairs, max_col, max_nodes;
+    int lv0, E0, max_pairs, max_col;
     const int *fl_ptr, *fl_split, *fl_a, *fl_b;
-    const int *de_node, *de_p, *de_ext, *de_lo, *de_hi;
+    const int *de_node, *de_p, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi;
 };
 
-// bytes of dynamic LDS of the flat tail factorisation: the pair products of
-// one level (the second product only for column destinations) and the
-// level's updated pivots
+// bytes of LDS of the flat factorisation: the pair products of one level
+// (the second product only for column destinations)
 template <typename T>
-inline long long flat_tail_lds(int max_pairs, int max_col, int max_nodes) {
-    return (long long)sizeof(T) * 4 * ((long long)max_pairs + max_col + max_nodes);
+inline long long flat_tail_lds(int max_pairs, int max_col) {
+    return (long long)sizeof(T) * 4 * ((long long)max_pairs + max_col);
 }
 
 // bytes of dynamic LDS of the tail sweeps: x of the tail nodes (2 doubles

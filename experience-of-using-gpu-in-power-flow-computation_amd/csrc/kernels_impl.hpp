@@ -317,6 +317,138 @@ __device__ void chunk_factor_levels(const DevSym &Y, T *blk, int lv0, int lv1, i
     }
 }
 
+// Sum of the blocks [lo, hi) of an LDS array in index order; W loads are
+// in flight before their additions, which keep the sequential order.
+template <typename T, int W>
+__device__ __forceinline__ B4<T> lds_sum_in_order(const T *v, int lo, int hi) {
+    B4<T> a{0, 0, 0, 0};
+    int q = lo;
+    for (; q + W <= hi; q += W) {
+        B4<T> x[W];
+#pragma unroll
+        for (int r = 0; r < W; r++) x[r] = load_b4(v, q + r);
+#pragma unroll
+        for (int r = 0; r < W; r++) a = B4<T>{a.a + x[r].a, a.b + x[r].b, a.c + x[r].c, a.d + x[r].d};
+    }
+    for (; q < hi; q++) {
+        const B4<T> x = load_b4(v, q);
+        a = B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
+    }
+    return a;
+}
+
+// Levels [F.lv0, lv1) of a flat pair program (DevFlat) in one workgroup,
+// two barriers per level:
+//   A. one update pair per lane (strided), its block product(s) into LDS
+//      (s_pa; s_pu for the column destinations' second product);
+//   B. one thread per destination adds its products in pair order; a pivot
+//      thread stores the inverse of the updated pivot, a column thread
+//      forms the same updated pivot from the pivot's products, inverts it
+//      and stores its L block scaled by it (both L copies) and its U block.
+//      The pivot slots are left as they were (nothing after the
+//      factorisation reads them; the inverse pivots carry the result).
+// Only A waits on global loads, one round of them: a lane's first R pairs
+// of a level have their indices in registers from the level before, and
+// all their blocks are requested before the first product (a pair past the
+// level's end reads block 0 and is not stored).  B sums W products per
+// round of LDS loads.  s_pa and s_pu hold F.max_pairs and F.max_col blocks.
+template <typename T, int R, int W>
+__device__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, int lv1, int *bad, T *s_pa, T *s_pu) {
+    const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, lv0 = F.lv0;
+    int pa[R], pb[R];
+    auto pairs_at = [&](int t, int *qa, int *qb) __attribute__((always_inline)) {
+        const int b = F.fl_ptr[t], np = F.fl_ptr[t + 1] - b;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int e = tid + r * bd;
+            qa[r] = e < np ? F.fl_a[b + e] : 0;
+            qb[r] = e < np ? F.fl_b[b + e] : 0;
+        }
+    };
+    struct Dest {
+        int j, p, lo, hi, dlo, dhi, lrp;
+        B4<T> v0, v1, vd;   // current blocks: own (L or pivot), U, and the node's pivot
+    };
+    auto dest = [&](int E) __attribute__((always_inline)) {
+        Dest D;
+        D.j = F.de_node[E];
+        D.p = F.de_p[E];
+        D.lo = F.de_lo[E];
+        D.hi = F.de_hi[E];
+        D.dlo = F.de_dlo[E];
+        D.dhi = F.de_dhi[E];
+        D.vd = load_b4(blk, D.j);
+        D.v0 = D.vd;
+        D.v1 = B4<T>{0, 0, 0, 0};
+        D.lrp = 0;
+        if (D.p >= 0) {
+            D.v0 = load_b4(blk, L0 + D.p);
+            D.v1 = load_b4(blk, U0 + D.p);
+            D.lrp = Y.lr_pos[D.p];
+        }
+        return D;
+    };
+    if (lv0 < lv1) pairs_at(0, pa, pb);
+    for (int l = lv0; l < lv1; l++) {
+        const int t = l - lv0, base = F.fl_ptr[t], np = F.fl_ptr[t + 1] - base, ns = F.fl_split[t];
+        const int e0 = Y.ld_ptr[l] - F.E0, ne = Y.ld_ptr[l + 1] - Y.ld_ptr[l];
+        Dest mine{};   // this thread's first destination, read during A
+        if (tid < ne) mine = dest(e0 + tid);
+        {
+            B4<T> la[R], ub[R], lb[R], ua[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                la[r] = load_b4(blk, L0 + pa[r]);
+                ub[r] = load_b4(blk, U0 + pb[r]);
+                lb[r] = load_b4(blk, L0 + pb[r]);
+                ua[r] = load_b4(blk, U0 + pa[r]);
+            }
+            if (l + 1 < lv1) pairs_at(t + 1, pa, pb);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int e = tid + r * bd;
+                if (e < np) {
+                    store_b4(s_pa, e, mul(la[r], ub[r]));
+                    if (e >= ns) store_b4(s_pu, e - ns, mul(lb[r], ua[r]));
+                }
+            }
+        }
+        for (int e = tid + R * bd; e < np; e += bd) {
+            const int qa = F.fl_a[base + e], qb = F.fl_b[base + e];
+            store_b4(s_pa, e, mul(load_b4(blk, L0 + qa), load_b4(blk, U0 + qb)));
+            if (e >= ns) store_b4(s_pu, e - ns, mul(load_b4(blk, L0 + qb), load_b4(blk, U0 + qa)));
+        }
+        __syncthreads();
+        for (int e = tid; e < ne; e += bd) {
+            const Dest D = (e == tid) ? mine : dest(e0 + e);
+            const B4<T> a = lds_sum_in_order<T, W>(s_pa, D.lo, D.hi);
+            B4<T> P;
+            if (D.p < 0) {
+                P = B4<T>{D.v0.a - a.a, D.v0.b - a.b, D.v0.c - a.c, D.v0.d - a.d};
+            } else {
+                const B4<T> ad = lds_sum_in_order<T, W>(s_pa, D.dlo, D.dhi);
+                P = B4<T>{D.vd.a - ad.a, D.vd.b - ad.b, D.vd.c - ad.c, D.vd.d - ad.d};
+            }
+            T det = P.a * P.d - P.b * P.c;
+            const bool sing = !(det != (T)0) || !isfinite((double)det);
+            if (sing) det = (T)1;
+            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
+            if (D.p < 0) {   // the pivot slot keeps its value: B of a later round reads it
+                if (sing) atomicOr(bad, 1);
+                store_b4(blk, DI + D.j, Pi);
+            } else {
+                const B4<T> u = lds_sum_in_order<T, W>(s_pu, D.lo - ns, D.hi - ns);
+                const B4<T> lb = mul(B4<T>{D.v0.a - a.a, D.v0.b - a.b, D.v0.c - a.c, D.v0.d - a.d}, Pi);
+                store_b4(blk, L0 + D.p, lb);
+                store_b4(blk, LR0 + D.lrp, lb);
+                store_b4(blk, U0 + D.p, B4<T>{D.v1.a - u.a, D.v1.b - u.b, D.v1.c - u.c, D.v1.d - u.d});
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------- trailing dense chain
 // The top of a power-network elimination tree ends in a chain of single
 // nodes whose columns are full (the separator between the network's areas).

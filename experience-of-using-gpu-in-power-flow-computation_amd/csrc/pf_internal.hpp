@@ -144,22 +144,25 @@ struct TailSweeps {
 };
 TailSweeps tail_sweeps(const Symbolic &s);
 
-// The tail levels' factorisation with flat pair lists (single-network
-// solver): the update pairs from sources below the tail (k < t0) are applied
-// first by a grid-wide pass, the rest level by level, one pair per lane.
-// Per tail destination E (flattened ld entry minus ld_ptr[tail_start]):
-// node de_node[E], L position de_p[E] (-1: the pivot), its pairs from
-// sources below the tail fc_ptr[d] .. de_ext[E], and its pairs from tail
-// sources at [de_lo[E], de_hi[E]) of its level's flat list.  Level
-// tail_start + t owns fl_a/fl_b[fl_ptr[t], fl_ptr[t+1]) (pairs (p_ik, p_jk)),
-// the pivots' pairs first, the column destinations' from fl_ptr[t] +
-// fl_split[t] on; max_pairs / max_col / max_nodes bound one level.
+// Levels [lv0, nlev_s) factored as flat pair lists, one pair per lane: with
+// split_below (single-network solver, lv0 = tail_start) the update pairs
+// from sources below lv0 (k < t0 = lvl_ptr[lv0]) are applied first by a
+// grid-wide pass and the lists hold the rest; without it (one workgroup per
+// scenario) the lists hold every pair.  Per destination E (flattened ld
+// entry minus ld_ptr[lv0]): node de_node[E], L position de_p[E] (-1: the
+// pivot), its pairs from sources below t0 fc_ptr[d] .. de_ext[E], its pairs
+// at [de_lo[E], de_hi[E]) of its level's flat list, and those of its node's
+// pivot at [de_dlo[E], de_dhi[E]).  Level lv0 + t owns fl_a/fl_b[fl_ptr[t],
+// fl_ptr[t+1]) (pairs (p_ik, p_jk)), the pivots' pairs first, the column
+// destinations' from fl_ptr[t] + fl_split[t] on; max_pairs / max_col bound
+// one level.
 struct FlatTail {
+    int lv0 = 0;
     std::vector<int> fl_ptr, fl_split, fl_a, fl_b;
-    std::vector<int> de_node, de_p, de_ext, de_lo, de_hi;
-    int max_pairs = 0, max_col = 0, max_nodes = 0;
+    std::vector<int> de_node, de_p, de_ext, de_lo, de_hi, de_dlo, de_dhi;
+    int max_pairs = 0, max_col = 0;
 };
-FlatTail flat_tail(const Symbolic &s);
+FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below);
 
 // Splits the levels below s.tail_start into at most max_parts groups of
 // whole subtrees, balanced by factorisation work (longest processing time

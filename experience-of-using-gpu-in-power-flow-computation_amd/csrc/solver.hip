@@ -431,12 +431,12 @@ pf_status pf_solver::setup_tail_sweeps() {
 // The tail levels' factorisation with flat pair lists, per precision as
 // the LDS holds one level's products.
 pf_status pf_solver::setup_flat_tail() {
-    const FlatTail ft = flat_tail(sym);
+    const FlatTail ft = flat_tail(sym, sym.tail_start, true);
     flat_ne = (int)ft.de_node.size();
-    dfl.E0 = sym.ld_ptr[sym.tail_start];
+    dfl.lv0 = ft.lv0;
+    dfl.E0 = sym.ld_ptr[ft.lv0];
     dfl.max_pairs = ft.max_pairs;
     dfl.max_col = ft.max_col;
-    dfl.max_nodes = ft.max_nodes;
     PFCHK(upload((int **)&dfl.fl_ptr, ft.fl_ptr));
     PFCHK(upload((int **)&dfl.fl_split, ft.fl_split));
     PFCHK(upload((int **)&dfl.fl_a, ft.fl_a));
@@ -446,20 +446,22 @@ pf_status pf_solver::setup_flat_tail() {
     PFCHK(upload((int **)&dfl.de_ext, ft.de_ext));
     PFCHK(upload((int **)&dfl.de_lo, ft.de_lo));
     PFCHK(upload((int **)&dfl.de_hi, ft.de_hi));
+    PFCHK(upload((int **)&dfl.de_dlo, ft.de_dlo));
+    PFCHK(upload((int **)&dfl.de_dhi, ft.de_dhi));
     int max_lds = 0;
     HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     const long long cap = max_lds - 8192;
-    const long long b64 = flat_tail_lds<double>(ft.max_pairs, ft.max_col, ft.max_nodes);
-    const long long b32 = flat_tail_lds<float>(ft.max_pairs, ft.max_col, ft.max_nodes);
+    const long long b64 = std::max<long long>(flat_tail_lds<double>(ft.max_pairs, ft.max_col), 16);
+    const long long b32 = std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16);
     if (b64 <= cap) {
         HIPCHK(hipFuncSetAttribute((const void *)k_factor_tail_flat<double>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)b64));
-        flat_lds[0] = (int)std::max<long long>(b64, 16);
+        flat_lds[0] = (int)b64;
     }
     if (b32 <= cap) {
         HIPCHK(hipFuncSetAttribute((const void *)k_factor_tail_flat<float>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)b32));
-        flat_lds[1] = (int)std::max<long long>(b32, 16);
+        flat_lds[1] = (int)b32;
     }
     return PF_OK;
 }
@@ -482,7 +484,7 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     const int fl = sym.tail_start < sym.nlev_s ? flat_lds[sizeof(T) == 4 ? 1 : 0] : 0;
     if (fl) {   // narrow levels below the chain: pairs from below the tail across the GPU first
         k_tail_factor_ext<T><<<(flat_ne + 31) / 32, 256, 0, stream>>>(dy, dfl, blk, ctl, flat_ne, gate);
-        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.tail_start, sym.nlev_s, gate);
+        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.nlev_s, gate);
     } else if (sym.tail_start < sym.nlev_s) {
         k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev_s, gate);
     }

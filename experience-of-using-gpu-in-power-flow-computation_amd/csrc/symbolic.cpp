@@ -431,20 +431,18 @@ TailSweeps tail_sweeps(const Symbolic &s) {
     return t;
 }
 
-FlatTail flat_tail(const Symbolic &s) {
+FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below) {
     FlatTail f;
-    const int t0 = s.lvl_ptr[s.tail_start], E0 = s.ld_ptr[s.tail_start];
+    f.lv0 = lv0;
+    const int t0 = split_below ? s.lvl_ptr[lv0] : 0, E0 = s.ld_ptr[lv0];
     std::vector<int> pcol(s.nnzL);
     for (int k = 0; k < s.n; k++)
         for (int p = s.colptr[k]; p < s.colptr[k + 1]; p++) pcol[p] = k;
     const int ne = s.ld_ptr[s.nlev_s] - E0;
-    f.de_node.resize(ne);
-    f.de_p.resize(ne);
-    f.de_ext.resize(ne);
-    f.de_lo.resize(ne);
-    f.de_hi.resize(ne);
+    for (auto *v : {&f.de_node, &f.de_p, &f.de_ext, &f.de_lo, &f.de_hi, &f.de_dlo, &f.de_dhi}) v->resize(ne);
+    std::vector<int> dlo(s.n), dhi(s.n);   // the pivot's pair range of each node of the level
     f.fl_ptr.assign(1, 0);
-    for (int l = s.tail_start; l < s.nlev_s; l++) {
+    for (int l = lv0; l < s.nlev_s; l++) {
         const int base = f.fl_ptr.back();
         for (int pass = 0; pass < 2; pass++) {   // the pivots' pairs, then the column destinations'
             if (pass == 1) f.fl_split.push_back((int)f.fl_a.size() - base);
@@ -463,12 +461,17 @@ FlatTail flat_tail(const Symbolic &s) {
                     f.fl_b.push_back(s.fc_b[q]);
                 }
                 f.de_hi[E] = (int)f.fl_a.size() - base;
+                if (t == 0) {
+                    dlo[j] = f.de_lo[E];
+                    dhi[j] = f.de_hi[E];
+                }
+                f.de_dlo[E] = dlo[j];
+                f.de_dhi[E] = dhi[j];
             }
         }
         const int np = (int)f.fl_a.size() - base;
         f.max_pairs = std::max(f.max_pairs, np);
         f.max_col = std::max(f.max_col, np - f.fl_split.back());
-        f.max_nodes = std::max(f.max_nodes, s.lvl_ptr[l + 1] - s.lvl_ptr[l]);
         f.fl_ptr.push_back((int)f.fl_a.size());
     }
     return f;

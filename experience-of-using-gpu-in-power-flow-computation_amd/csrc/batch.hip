@@ -785,7 +785,8 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
             for (int l = 0; l < Y.nlev_s;) {
                 if (s_ck && tp.F.fl_ptr && l == tp.F.lv0 && ((size_t)w.x & 15) == 0) {   // flat pair lists
                     T *pa = reinterpret_cast<T *>(w.x);   // x and r: free until the solve
-                    flat_factor_levels<T, 1, 4>(Y, tp.F, w.blk, Y.nlev_s, &s_sing, pa, pa + 4 * (size_t)tp.F.max_pairs);
+                    flat_factor_levels<T, 1, 4>(Y, tp.F, w.blk, 0, Y.nlev_s - tp.F.lv0, &s_sing, pa,
+                                                pa + 4 * (size_t)tp.F.max_pairs);
                     break;
                 }
                 if (s_ck && Y.ck_lvl && l >= Y.tail_start) {   // the chunked narrow top
@@ -1265,15 +1266,16 @@ pf_status pf_batch::add_net(const pf_network &src) {
         if (need(ft) <= 32LL * sym.n) {
             DevFlat &F = b->tp.F;
             F.lv0 = ft.lv0;
-            F.E0 = sym.ld_ptr[ft.lv0];
             F.max_pairs = ft.max_pairs;
             F.max_col = ft.max_col;
             BCHK(upload((int **)&F.fl_ptr, ft.fl_ptr));
             BCHK(upload((int **)&F.fl_split, ft.fl_split));
             BCHK(upload((int **)&F.fl_a, ft.fl_a));
             BCHK(upload((int **)&F.fl_b, ft.fl_b));
+            BCHK(upload((int **)&F.de_ptr, ft.de_ptr));
             BCHK(upload((int **)&F.de_node, ft.de_node));
             BCHK(upload((int **)&F.de_p, ft.de_p));
+            BCHK(upload((int **)&F.de_q0, ft.de_q0));
             BCHK(upload((int **)&F.de_ext, ft.de_ext));
             BCHK(upload((int **)&F.de_lo, ft.de_lo));
             BCHK(upload((int **)&F.de_hi, ft.de_hi));

@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(256) k_tail_factor_ext(DevSym Y, DevFlat F, T 
     const int E = (int)blockIdx.x * (256 / G) + (int)threadIdx.x / G, lane = (int)threadIdx.x % G;
     if (E >= ne) return;                                 // whole groups leave together
     const int L0 = Y.n, U0 = Y.n + Y.nnzL;
-    const int d = Y.ld_d[F.E0 + E], p = F.de_p[E], q0 = Y.fc_ptr[d], q1 = F.de_ext[E];
+    const int p = F.de_p[E], q0 = F.de_q0[E], q1 = F.de_ext[E];
     B4<T> a{0, 0, 0, 0}, u{0, 0, 0, 0};
     if (p < 0) pairs_acc<T, false, 2>(Y, blk, q0 + lane, q1, G, a, u);
     else pairs_acc<T, true, 2>(Y, blk, q0 + lane, q1, G, a, u);
@@ -375,12 +375,24 @@ __global__ void __launch_bounds__(256) k_tail_factor_ext(DevSym Y, DevFlat F, T 
 // Flat tail factorisation, second part: the tail levels in one workgroup
 // (flat_factor_levels), the level's products in dynamic LDS.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int lv1, int gate) {
+__global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int nsteps, int gate) {
     if (gate && !ctl->step) return;
     extern __shared__ __align__(16) char smem[];
     T *s_pa = reinterpret_cast<T *>(smem);
-    flat_factor_levels<T, sizeof(T) == 4 ? 3 : 2, sizeof(T) == 4 ? 8 : 4>(Y, F, blk, lv1, &ctl->singular, s_pa,
+    flat_factor_levels<T, sizeof(T) == 4 ? 3 : 2, sizeof(T) == 4 ? 8 : 4>(Y, F, blk, 0, nsteps, &ctl->singular, s_pa,
                                                                             s_pa + 4 * (size_t)F.max_pairs);
+}
+
+// Levels below the tail as flat pair programs: workgroup g walks the level
+// steps of its group of subtrees (g * part_levels .. + part_levels).
+template <typename T>
+__global__ void __launch_bounds__(256) k_factor_parts_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int gate) {
+    if (gate && !ctl->step) return;
+    extern __shared__ __align__(16) char smem[];
+    T *s_pa = reinterpret_cast<T *>(smem);
+    const int t0 = (int)blockIdx.x * Y.part_levels;
+    flat_factor_levels<T, 2, 4>(Y, F, blk, t0, t0 + Y.part_levels, &ctl->singular, s_pa,
+                                s_pa + 4 * (size_t)F.max_pairs);
 }
 
 // Schur complement of the trailing dense chain: every destination is
@@ -1041,6 +1053,8 @@ template __global__ void k_tail_factor_ext<double>(DevSym, DevFlat, double *, Ct
 template __global__ void k_tail_factor_ext<float>(DevSym, DevFlat, float *, Ctl *, int, int);
 template __global__ void k_factor_tail_flat<double>(DevSym, DevFlat, double *, Ctl *, int, int);
 template __global__ void k_factor_tail_flat<float>(DevSym, DevFlat, float *, Ctl *, int, int);
+template __global__ void k_factor_parts_flat<double>(DevSym, DevFlat, double *, Ctl *, int);
+template __global__ void k_factor_parts_flat<float>(DevSym, DevFlat, float *, Ctl *, int);
 template __global__ void k_fwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);
 template __global__ void k_fwd_level<float>(DevSym, const float *, double *, Ctl *, int, int, int);
 template __global__ void k_bwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);

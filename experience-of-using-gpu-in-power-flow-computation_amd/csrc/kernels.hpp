@@ -114,12 +114,13 @@ struct DevTail {
     const int *bx_ptr, *bx_i, *bx_p, *bi_ptr, *bi_i, *bi_p;
 };
 
-// Levels [lv0, nlev_s) factored as flat pair lists (FlatTail on the host);
-// E0 = ld_ptr[lv0].
+// A flat pair program on the device (FlatTail on the host): level steps,
+// each its destinations [de_ptr[t], de_ptr[t+1]) and pairs
+// [fl_ptr[t], fl_ptr[t+1]); lv0: the first level of a level-by-level program.
 struct DevFlat {
-    int lv0, E0, max_pairs, max_col;
-    const int *fl_ptr, *fl_split, *fl_a, *fl_b;
-    const int *de_node, *de_p, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi;
+    int lv0, max_pairs, max_col;
+    const int *fl_ptr, *fl_split, *fl_a, *fl_b, *de_ptr;
+    const int *de_node, *de_p, *de_q0, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi;
 };
 
 // bytes of LDS of the flat factorisation: the pair products of one level

@@ -337,8 +337,8 @@ __device__ __forceinline__ B4<T> lds_sum_in_order(const T *v, int lo, int hi) {
     return a;
 }
 
-// Levels [F.lv0, lv1) of a flat pair program (DevFlat) in one workgroup,
-// two barriers per level:
+// Level steps [t0, t1) of a flat pair program (DevFlat) in one workgroup,
+// two barriers per step:
 //   A. one update pair per lane (strided), its block product(s) into LDS
 //      (s_pa; s_pu for the column destinations' second product);
 //   B. one thread per destination adds its products in pair order; a pivot
@@ -348,14 +348,15 @@ __device__ __forceinline__ B4<T> lds_sum_in_order(const T *v, int lo, int hi) {
 //      The pivot slots are left as they were (nothing after the
 //      factorisation reads them; the inverse pivots carry the result).
 // Only A waits on global loads, one round of them: a lane's first R pairs
-// of a level have their indices in registers from the level before, and
-// all their blocks are requested before the first product (a pair past the
-// level's end reads block 0 and is not stored).  B sums W products per
-// round of LDS loads.  s_pa and s_pu hold F.max_pairs and F.max_col blocks.
+// of a step have their indices in registers from the step before, and all
+// their blocks are requested before the first product (a pair past the
+// step's end reads block 0 and is not stored).  B sums W products per round
+// of LDS loads.  s_pa and s_pu hold F.max_pairs and F.max_col blocks.
 template <typename T, int R, int W>
-__device__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, int lv1, int *bad, T *s_pa, T *s_pu) {
+__device__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, int t0, int t1, int *bad, T *s_pa,
+                                   T *s_pu) {
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
-    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, lv0 = F.lv0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
     int pa[R], pb[R];
     auto pairs_at = [&](int t, int *qa, int *qb) __attribute__((always_inline)) {
         const int b = F.fl_ptr[t], np = F.fl_ptr[t + 1] - b;
@@ -389,10 +390,10 @@ __device__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, in
         }
         return D;
     };
-    if (lv0 < lv1) pairs_at(0, pa, pb);
-    for (int l = lv0; l < lv1; l++) {
-        const int t = l - lv0, base = F.fl_ptr[t], np = F.fl_ptr[t + 1] - base, ns = F.fl_split[t];
-        const int e0 = Y.ld_ptr[l] - F.E0, ne = Y.ld_ptr[l + 1] - Y.ld_ptr[l];
+    if (t0 < t1) pairs_at(t0, pa, pb);
+    for (int t = t0; t < t1; t++) {
+        const int base = F.fl_ptr[t], np = F.fl_ptr[t + 1] - base, ns = F.fl_split[t];
+        const int e0 = F.de_ptr[t], ne = F.de_ptr[t + 1] - e0;
         Dest mine{};   // this thread's first destination, read during A
         if (tid < ne) mine = dest(e0 + tid);
         {
@@ -404,7 +405,7 @@ __device__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, in
                 lb[r] = load_b4(blk, L0 + pb[r]);
                 ua[r] = load_b4(blk, U0 + pa[r]);
             }
-            if (l + 1 < lv1) pairs_at(t + 1, pa, pb);
+            if (t + 1 < t1) pairs_at(t + 1, pa, pb);
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int e = tid + r * bd;

@@ -144,25 +144,31 @@ struct TailSweeps {
 };
 TailSweeps tail_sweeps(const Symbolic &s);
 
-// Levels [lv0, nlev_s) factored as flat pair lists, one pair per lane: with
-// split_below (single-network solver, lv0 = tail_start) the update pairs
-// from sources below lv0 (k < t0 = lvl_ptr[lv0]) are applied first by a
-// grid-wide pass and the lists hold the rest; without it (one workgroup per
-// scenario) the lists hold every pair.  Per destination E (flattened ld
-// entry minus ld_ptr[lv0]): node de_node[E], L position de_p[E] (-1: the
-// pivot), its pairs from sources below t0 fc_ptr[d] .. de_ext[E], its pairs
-// at [de_lo[E], de_hi[E]) of its level's flat list, and those of its node's
-// pivot at [de_dlo[E], de_dhi[E]).  Level lv0 + t owns fl_a/fl_b[fl_ptr[t],
+// Levels factored as flat pair lists, one pair per lane.  A program is a
+// sequence of level steps t: step t owns the destinations
+// [de_ptr[t], de_ptr[t+1]) and the update pairs fl_a/fl_b[fl_ptr[t],
 // fl_ptr[t+1]) (pairs (p_ik, p_jk)), the pivots' pairs first, the column
-// destinations' from fl_ptr[t] + fl_split[t] on; max_pairs / max_col bound
-// one level.
+// destinations' from fl_ptr[t] + fl_split[t] on.  Destination E: node
+// de_node[E], L position de_p[E] (-1: the pivot), its pairs at
+// [de_lo[E], de_hi[E]) of its step's list and those of its node's pivot at
+// [de_dlo[E], de_dhi[E]); its pairs from sources below the program, applied
+// before it, are fc_a/fc_b[de_q0[E], de_ext[E]).  max_pairs / max_col bound
+// one step.
+//   flat_tail: levels [lv0, nlev_s), step t = level lv0 + t, destinations
+//   in ld order; with split_below (single-network solver, lv0 = tail_start)
+//   the pairs from sources below lv0 (k < lvl_ptr[lv0]) are left to a
+//   grid-wide pass, without it (one workgroup per scenario) the lists hold
+//   every pair.
+//   flat_parts: the subtree groups below tail_start, step g * tail_start + l
+//   = the nodes of group g on level l (every pair is inside the group).
 struct FlatTail {
     int lv0 = 0;
-    std::vector<int> fl_ptr, fl_split, fl_a, fl_b;
-    std::vector<int> de_node, de_p, de_ext, de_lo, de_hi, de_dlo, de_dhi;
+    std::vector<int> fl_ptr, fl_split, fl_a, fl_b, de_ptr;
+    std::vector<int> de_node, de_p, de_q0, de_ext, de_lo, de_hi, de_dlo, de_dhi;
     int max_pairs = 0, max_col = 0;
 };
 FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below);
+FlatTail flat_parts(const Symbolic &s);
 
 // Splits the levels below s.tail_start into at most max_parts groups of
 // whole subtrees, balanced by factorisation work (longest processing time

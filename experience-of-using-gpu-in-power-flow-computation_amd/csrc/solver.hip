@@ -66,9 +66,9 @@ struct pf_solver {
     DevTail dtl{};
     // flat tail factorisation (blocked program): per precision [double,
     // float] its dynamic LDS bytes, 0 = not used (the chunked walk)
-    DevFlat dfl{};
+    DevFlat dfl{}, dfp{};
     int flat_ne = 0;
-    int flat_lds[2] = {0, 0};
+    int flat_lds[2] = {0, 0}, parts_lds[2] = {0, 0};
     int tail_mode[2] = {0, 0};
     int tail_lds_f[2] = {0, 0}, tail_lds_b[2] = {0, 0};
     double *blk64 = nullptr;
@@ -154,6 +154,7 @@ struct pf_solver {
     pf_status enqueue_factor_solve(int gate);
     pf_status setup_tail_sweeps();
     pf_status setup_flat_tail();
+    pf_status upload_flat(const FlatTail &ft, DevFlat &F);
     template <typename T> pf_status enqueue_lu(T *blk, int gate);
     template <typename T> pf_status enqueue_solve(const T *blk, double *v, int gate);
     pf_status enqueue_step(int gate);
@@ -430,38 +431,52 @@ pf_status pf_solver::setup_tail_sweeps() {
 
 // The tail levels' factorisation with flat pair lists, per precision as
 // the LDS holds one level's products.
+pf_status pf_solver::upload_flat(const FlatTail &ft, DevFlat &F) {
+    F.lv0 = ft.lv0;
+    F.max_pairs = ft.max_pairs;
+    F.max_col = ft.max_col;
+    PFCHK(upload((int **)&F.fl_ptr, ft.fl_ptr));
+    PFCHK(upload((int **)&F.fl_split, ft.fl_split));
+    PFCHK(upload((int **)&F.fl_a, ft.fl_a));
+    PFCHK(upload((int **)&F.fl_b, ft.fl_b));
+    PFCHK(upload((int **)&F.de_ptr, ft.de_ptr));
+    PFCHK(upload((int **)&F.de_node, ft.de_node));
+    PFCHK(upload((int **)&F.de_p, ft.de_p));
+    PFCHK(upload((int **)&F.de_q0, ft.de_q0));
+    PFCHK(upload((int **)&F.de_ext, ft.de_ext));
+    PFCHK(upload((int **)&F.de_lo, ft.de_lo));
+    PFCHK(upload((int **)&F.de_hi, ft.de_hi));
+    PFCHK(upload((int **)&F.de_dlo, ft.de_dlo));
+    PFCHK(upload((int **)&F.de_dhi, ft.de_dhi));
+    return PF_OK;
+}
+
+// The tail levels and the subtree groups as flat pair programs, per
+// precision as the LDS holds one step's products.
 pf_status pf_solver::setup_flat_tail() {
-    const FlatTail ft = flat_tail(sym, sym.tail_start, true);
-    flat_ne = (int)ft.de_node.size();
-    dfl.lv0 = ft.lv0;
-    dfl.E0 = sym.ld_ptr[ft.lv0];
-    dfl.max_pairs = ft.max_pairs;
-    dfl.max_col = ft.max_col;
-    PFCHK(upload((int **)&dfl.fl_ptr, ft.fl_ptr));
-    PFCHK(upload((int **)&dfl.fl_split, ft.fl_split));
-    PFCHK(upload((int **)&dfl.fl_a, ft.fl_a));
-    PFCHK(upload((int **)&dfl.fl_b, ft.fl_b));
-    PFCHK(upload((int **)&dfl.de_node, ft.de_node));
-    PFCHK(upload((int **)&dfl.de_p, ft.de_p));
-    PFCHK(upload((int **)&dfl.de_ext, ft.de_ext));
-    PFCHK(upload((int **)&dfl.de_lo, ft.de_lo));
-    PFCHK(upload((int **)&dfl.de_hi, ft.de_hi));
-    PFCHK(upload((int **)&dfl.de_dlo, ft.de_dlo));
-    PFCHK(upload((int **)&dfl.de_dhi, ft.de_dhi));
     int max_lds = 0;
     HIPCHK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     const long long cap = max_lds - 8192;
-    const long long b64 = std::max<long long>(flat_tail_lds<double>(ft.max_pairs, ft.max_col), 16);
-    const long long b32 = std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16);
-    if (b64 <= cap) {
-        HIPCHK(hipFuncSetAttribute((const void *)k_factor_tail_flat<double>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)b64));
-        flat_lds[0] = (int)b64;
-    }
-    if (b32 <= cap) {
-        HIPCHK(hipFuncSetAttribute((const void *)k_factor_tail_flat<float>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)b32));
-        flat_lds[1] = (int)b32;
+    auto allow = [&](const void *k, long long bytes, int &out) -> pf_status {
+        if (bytes > cap) return PF_OK;
+        HIPCHK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+        out = (int)bytes;
+        return PF_OK;
+    };
+    const FlatTail ft = flat_tail(sym, sym.tail_start, true);
+    flat_ne = (int)ft.de_node.size();
+    PFCHK(upload_flat(ft, dfl));
+    PFCHK(allow((const void *)k_factor_tail_flat<double>,
+                std::max<long long>(flat_tail_lds<double>(ft.max_pairs, ft.max_col), 16), flat_lds[0]));
+    PFCHK(allow((const void *)k_factor_tail_flat<float>,
+                std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16), flat_lds[1]));
+    if (sym.n_parts > 0) {
+        const FlatTail fp = flat_parts(sym);
+        PFCHK(upload_flat(fp, dfp));
+        PFCHK(allow((const void *)k_factor_parts_flat<double>,
+                    std::max<long long>(flat_tail_lds<double>(fp.max_pairs, fp.max_col), 16), parts_lds[0]));
+        PFCHK(allow((const void *)k_factor_parts_flat<float>,
+                    std::max<long long>(flat_tail_lds<float>(fp.max_pairs, fp.max_col), 16), parts_lds[1]));
     }
     return PF_OK;
 }
@@ -473,7 +488,10 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     mark(2);
     k_jfill<T><<<(nfill + 255) / 256, 256, 0, stream>>>(dn, ds, dy, blk, gate);
     mark(3);
-    if (sym.n_parts > 0) {
+    const int pl = parts_lds[sizeof(T) == 4 ? 1 : 0];
+    if (sym.n_parts > 0 && pl) {
+        k_factor_parts_flat<T><<<sym.n_parts, 256, pl, stream>>>(dy, dfp, blk, ctl, gate);
+    } else if (sym.n_parts > 0) {
         k_factor_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, ctl, gate);
     } else {
         for (int l = 0; l < sym.tail_start; l++) {
@@ -484,7 +502,7 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     const int fl = sym.tail_start < sym.nlev_s ? flat_lds[sizeof(T) == 4 ? 1 : 0] : 0;
     if (fl) {   // narrow levels below the chain: pairs from below the tail across the GPU first
         k_tail_factor_ext<T><<<(flat_ne + 31) / 32, 256, 0, stream>>>(dy, dfl, blk, ctl, flat_ne, gate);
-        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.nlev_s, gate);
+        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.nlev_s - sym.tail_start, gate);
     } else if (sym.tail_start < sym.nlev_s) {
         k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev_s, gate);
     }

@@ -431,49 +431,85 @@ TailSweeps tail_sweeps(const Symbolic &s) {
     return t;
 }
 
-FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below) {
-    FlatTail f;
-    f.lv0 = lv0;
-    const int t0 = split_below ? s.lvl_ptr[lv0] : 0, E0 = s.ld_ptr[lv0];
-    std::vector<int> pcol(s.nnzL);
-    for (int k = 0; k < s.n; k++)
-        for (int p = s.colptr[k]; p < s.colptr[k + 1]; p++) pcol[p] = k;
-    const int ne = s.ld_ptr[s.nlev_s] - E0;
-    for (auto *v : {&f.de_node, &f.de_p, &f.de_ext, &f.de_lo, &f.de_hi, &f.de_dlo, &f.de_dhi}) v->resize(ne);
-    std::vector<int> dlo(s.n), dhi(s.n);   // the pivot's pair range of each node of the level
-    f.fl_ptr.assign(1, 0);
-    for (int l = lv0; l < s.nlev_s; l++) {
-        const int base = f.fl_ptr.back();
-        for (int pass = 0; pass < 2; pass++) {   // the pivots' pairs, then the column destinations'
-            if (pass == 1) f.fl_split.push_back((int)f.fl_a.size() - base);
-            for (int e = s.ld_ptr[l]; e < s.ld_ptr[l + 1]; e++) {
-                const int j = s.ld_node[e], d = s.ld_d[e], t = d - s.fd_ptr[j];
+namespace {
+
+// one level step of a flat program: the destinations of the given nodes,
+// the pivots' pairs first (pairs from sources k < t0 left out)
+void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0, const std::vector<int> &pcol,
+               std::vector<int> &dlo, std::vector<int> &dhi) {
+    const int base = f.fl_ptr.back();
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) f.fl_split.push_back((int)f.fl_a.size() - base);
+        for (int a = 0; a < nn; a++) {
+            const int j = nodes[a];
+            for (int d = s.fd_ptr[j]; d < s.fd_ptr[j + 1]; d++) {
+                const int t = d - s.fd_ptr[j];
                 if ((t == 0) != (pass == 0)) continue;
                 int q = s.fc_ptr[d];
                 while (q < s.fc_ptr[d + 1] && pcol[s.fc_b[q]] < t0) q++;   // pairs ascend in k
-                const int E = e - E0;
-                f.de_node[E] = j;
-                f.de_p[E] = t == 0 ? -1 : s.colptr[j] + t - 1;
-                f.de_ext[E] = q;
-                f.de_lo[E] = (int)f.fl_a.size() - base;
+                f.de_node.push_back(j);
+                f.de_p.push_back(t == 0 ? -1 : s.colptr[j] + t - 1);
+                f.de_q0.push_back(s.fc_ptr[d]);
+                f.de_ext.push_back(q);
+                f.de_lo.push_back((int)f.fl_a.size() - base);
                 for (; q < s.fc_ptr[d + 1]; q++) {
                     f.fl_a.push_back(s.fc_a[q]);
                     f.fl_b.push_back(s.fc_b[q]);
                 }
-                f.de_hi[E] = (int)f.fl_a.size() - base;
+                f.de_hi.push_back((int)f.fl_a.size() - base);
                 if (t == 0) {
-                    dlo[j] = f.de_lo[E];
-                    dhi[j] = f.de_hi[E];
+                    dlo[j] = f.de_lo.back();
+                    dhi[j] = f.de_hi.back();
                 }
-                f.de_dlo[E] = dlo[j];
-                f.de_dhi[E] = dhi[j];
+                f.de_dlo.push_back(dlo[j]);
+                f.de_dhi.push_back(dhi[j]);
             }
         }
-        const int np = (int)f.fl_a.size() - base;
-        f.max_pairs = std::max(f.max_pairs, np);
-        f.max_col = std::max(f.max_col, np - f.fl_split.back());
-        f.fl_ptr.push_back((int)f.fl_a.size());
     }
+    const int np = (int)f.fl_a.size() - base;
+    f.max_pairs = std::max(f.max_pairs, np);
+    f.max_col = std::max(f.max_col, np - f.fl_split.back());
+    f.fl_ptr.push_back((int)f.fl_a.size());
+    f.de_ptr.push_back((int)f.de_node.size());
+}
+
+std::vector<int> position_columns(const Symbolic &s) {
+    std::vector<int> pcol(s.nnzL);
+    for (int k = 0; k < s.n; k++)
+        for (int p = s.colptr[k]; p < s.colptr[k + 1]; p++) pcol[p] = k;
+    return pcol;
+}
+
+}  // namespace
+
+FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below) {
+    FlatTail f;
+    f.lv0 = lv0;
+    const int t0 = split_below ? s.lvl_ptr[lv0] : 0;
+    const std::vector<int> pcol = position_columns(s);
+    std::vector<int> dlo(s.n), dhi(s.n);
+    f.fl_ptr.assign(1, 0);
+    f.de_ptr.assign(1, 0);
+    for (int l = lv0; l < s.nlev_s; l++) {   // nodes are numbered level by level
+        std::vector<int> nodes;
+        for (int j = s.lvl_ptr[l]; j < s.lvl_ptr[l + 1]; j++) nodes.push_back(j);
+        flat_step(s, f, nodes.data(), (int)nodes.size(), t0, pcol, dlo, dhi);
+    }
+    return f;
+}
+
+FlatTail flat_parts(const Symbolic &s) {
+    FlatTail f;
+    const int T = s.tail_start;
+    const std::vector<int> pcol = position_columns(s);
+    std::vector<int> dlo(s.n), dhi(s.n);
+    f.fl_ptr.assign(1, 0);
+    f.de_ptr.assign(1, 0);
+    for (int g = 0; g < s.n_parts; g++)
+        for (int l = 0; l < T; l++) {
+            const int a = s.part_ptr[(size_t)g * T + l], b = s.part_ptr[(size_t)g * T + l + 1];
+            flat_step(s, f, s.part_nodes.data() + a, b - a, 0, pcol, dlo, dhi);
+        }
     return f;
 }
 

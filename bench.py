@@ -7,19 +7,26 @@ FP64 refinement, flat start), launched as one captured hipGraph of persistent
 kernels.  value = converged solves / second over the timed steps.
 
 Printed beside it, in the same JSON line:
+  * per-topology histograms of NR passes per scenario, and each topology's
+    persistent kernel launched alone (the roofline's dominant kernel);
   * plain FP64 GPU NR on the same 118-bus scenarios, one scenario at a time
-    (eager launches, one host sync per iteration, the sparse LU one
-    elimination-tree level at a time) and the batched/plain ratio;
-  * the single 15,104-bus solve: that plain FP64 path vs the optimised one
-    (dense separator block, one captured graph per iteration) in FP64 and
-    with the FP32 factor + FP64 refinement;
+    (the FP64 NR loop as one captured graph per iteration, the sparse LU on
+    the straightforward level schedule, lu_blocked = 0) and the
+    batched/plain ratio;
+  * the single 15,104-bus solve: that plain FP64 path against the optimised
+    schedule (lu_blocked = 1) in FP64 and with the FP32 factor + FP64
+    refinement, every variant one captured graph per iteration;
   * the learned warm start (944-bus network and the headline 118-bus
     scenarios): training on the solver's own answers, iterations and
-    solves/s against the flat start;
-  * roofline of the dominant persistent kernel from its HIP-event time;
+    solves/s against the flat start, the warm answers checked against the
+    flat-start oracle;
+  * roofline of the dominant persistent kernel from its HIP-event time, and
+    its HBM traffic from the committed rocprofv3 PMC summary when that was
+    taken on this source stamp with these arguments;
   * the oracle (FP64 C restatement of the reference CPU path) timed on the
-    host cores over the same workload, and the accuracy of the GPU answers
-    against it.
+    host cores over the same workload, the accuracy of the GPU answers and
+    their PV->PQ sets against it, and the single-precision (SPFP) reference
+    point.
 """
 from __future__ import annotations
 
@@ -375,6 +382,13 @@ def main():
     ms_step = 1e3 * wall / args.steps
 
     extras = {}
+    # per-topology histogram of NR passes per scenario: a launch lasts as long
+    # as its slowest scenario (the non-converging ones run to no_max_iter)
+    hist = {}
+    for name, its in (("1888-bus", IT[:args.n1888]), ("118-bus", IT[args.n1888:])):
+        vals, cnt = np.unique(its, return_counts=True)
+        hist[name] = {str(int(v)): int(c) for v, c in zip(vals, cnt)}
+    extras["iteration_histogram"] = hist
     spfp_gpu = None
     # roofline of each persistent kernel from its own timed launch
     roof = None

@@ -428,43 +428,109 @@ __device__ __forceinline__ double2 prod_sum_in_order(const double2 *v, int lo, i
     return make_double2(a0, a1);
 }
 
-// Forward and backward sweeps with one entry per lane: on every level each
-// lane forms one product L_jk x_k (backward U_ji x_i) into LDS (prod, page
-// of them at a time) and then one thread per node subtracts its products in
-// entry order (backward: and applies the inverse pivot after the level's
-// last page).  The entries of a level are contiguous (rs_ptr / colptr in
-// node order, nodes numbered level by level), so the entry a lane takes
-// first on the next level, its index and factor block (and the inverse
-// pivot of its first node), are loaded a level ahead: a level that fits one
-// page and one entry per lane waits on no global load.
+// Forward and backward sweeps whose factor entries are loaded a level
+// ahead (they do not depend on x).  The entries of a level are contiguous
+// (rs_ptr / colptr in node order, nodes numbered level by level).
+//  * Narrow levels (at most kSweepGroupLevel nodes, a constant, so every
+//    launch shape sums alike): 16 lanes per node, lane i taking entries
+//    i, i + 16, ...; its first kSweepAhead entries come from registers
+//    filled during the level before.  One barrier per level.
+//  * Wide levels: one entry per lane, the product L_jk x_k (backward
+//    U_ji x_i) into LDS (prod, `page` at a time), then one thread per node
+//    subtracts its products in entry order (backward: and applies the
+//    inverse pivot after the level's last page).  Two barriers per page.
+constexpr int kSweepGroupLevel = 16, kSweepAhead = 2;
+
 template <typename T>
 __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, double *x, double *s_r, double2 *prod,
                                               int page) {
-    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / GL, lane = tid % GL;
     const int LR0 = Y.n + 2 * Y.nnzL, U0 = Y.n + Y.nnzL, nl = Y.nlev_s;
-    int pk = -1;
-    B4<T> pB{0, 0, 0, 0};
-    auto fetch_f = [&](int l) __attribute__((always_inline)) {
-        const int e = Y.rs_ptr[Y.lvl_ptr[l]] + tid;
-        pk = -1;
-        if (e < Y.rs_ptr[Y.lvl_ptr[l + 1]]) {
-            pk = Y.rs_k[e];
-            pB = load_b4(blk, LR0 + e);
+    constexpr int RA = kSweepAhead;
+    int pk[RA];
+    B4<T> pB[RA], pD{0, 0, 0, 0};
+    // the entries this lane takes first on level l: a narrow level's node
+    // grp (entries lane + GL r), a wide level's entry tid; ptr / idx / B0 are
+    // the row (forward) or column (backward) structure and its blocks
+    auto ahead = [&](int l, const int *ptr, const int *idx, int B0, bool diag) __attribute__((always_inline)) {
+        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+#pragma unroll
+        for (int r = 0; r < RA; r++) pk[r] = -1;
+        if (c <= kSweepGroupLevel) {
+            if (grp < c) {
+                const int j = s + grp, e1 = ptr[j + 1];
+#pragma unroll
+                for (int r = 0; r < RA; r++) {
+                    const int e = ptr[j] + lane + GL * r;
+                    if (e < e1) {
+                        pk[r] = idx[e];
+                        pB[r] = load_b4(blk, B0 + e);
+                    }
+                }
+                if (diag && lane == 0) pD = load_b4(blk, Y.di0 + j);
+            }
+        } else {
+            const int e = ptr[s] + tid;
+            if (e < ptr[s + c]) {
+                pk[0] = idx[e];
+                pB[0] = load_b4(blk, B0 + e);
+            }
+            if (diag && tid < c) pD = load_b4(blk, Y.di0 + s + tid);
         }
     };
-    if (nl > 0) fetch_f(0);
+    // ---- forward
+    if (nl > 0) ahead(0, Y.rs_ptr, Y.rs_k, LR0, false);
     for (int l = 0; l < nl; l++) {
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        int ck[RA];
+        B4<T> cB[RA];
+#pragma unroll
+        for (int r = 0; r < RA; r++) {
+            ck[r] = pk[r];
+            cB[r] = pB[r];
+        }
+        if (c <= kSweepGroupLevel) {
+            if (l + 1 < nl) ahead(l + 1, Y.rs_ptr, Y.rs_k, LR0, false);
+            if (grp < c) {
+                const int j = s + grp;
+                double a0 = 0, a1 = 0;
+#pragma unroll
+                for (int r = 0; r < RA; r++)
+                    if (ck[r] >= 0) {
+                        const double x0 = x[2 * ck[r]], x1 = x[2 * ck[r] + 1];
+                        a0 += (double)cB[r].a * x0 + (double)cB[r].b * x1;
+                        a1 += (double)cB[r].c * x0 + (double)cB[r].d * x1;
+                    }
+                for (int e = Y.rs_ptr[j] + lane + GL * RA; e < Y.rs_ptr[j + 1]; e += GL) {
+                    const B4<T> L = load_b4(blk, LR0 + e);
+                    const int k = Y.rs_k[e];
+                    const double x0 = x[2 * k], x1 = x[2 * k + 1];
+                    a0 += (double)L.a * x0 + (double)L.b * x1;
+                    a1 += (double)L.c * x0 + (double)L.d * x1;
+                }
+#pragma unroll
+                for (int off = GL / 2; off > 0; off >>= 1) {
+                    a0 += __shfl_xor(a0, off, GL);
+                    a1 += __shfl_xor(a1, off, GL);
+                }
+                if (lane == 0) {
+                    x[2 * j] -= a0;
+                    x[2 * j + 1] -= a1;
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         const int e_lo = Y.rs_ptr[s], e_hi = Y.rs_ptr[s + c];
-        if (e_lo == e_hi && l + 1 < nl) fetch_f(l + 1);
+        if (e_lo == e_hi && l + 1 < nl) ahead(l + 1, Y.rs_ptr, Y.rs_k, LR0, false);
         for (int pg = e_lo; pg < e_hi; pg += page) {
             const int pe = min(pg + page, e_hi);
             for (int e = pg + tid; e < pe; e += bd) {
                 int k;
                 B4<T> L;
                 if (e == e_lo + tid) {
-                    k = pk;
-                    L = pB;
+                    k = ck[0];
+                    L = cB[0];
                 } else {
                     k = Y.rs_k[e];
                     L = load_b4(blk, LR0 + e);
@@ -472,7 +538,7 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 const double x0 = x[2 * k], x1 = x[2 * k + 1];
                 prod[e - pg] = make_double2((double)L.a * x0 + (double)L.b * x1, (double)L.c * x0 + (double)L.d * x1);
             }
-            if (pg == e_lo && l + 1 < nl) fetch_f(l + 1);
+            if (pg == e_lo && l + 1 < nl) ahead(l + 1, Y.rs_ptr, Y.rs_k, LR0, false);
             __syncthreads();
             for (int i = tid; i < c; i += bd) {
                 const int j = s + i, lo = max(Y.rs_ptr[j], pg), hi = min(Y.rs_ptr[j + 1], pe);
@@ -486,24 +552,54 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
         }
     }
     if (Y.ch_m > 0) chain_solve<T, GL>(Y, blk, x, s_r);
-    int pi = -1;
-    B4<T> pD{0, 0, 0, 0};
-    auto fetch_b = [&](int l) __attribute__((always_inline)) {
-        const int s = Y.lvl_ptr[l], e = Y.colptr[s] + tid;
-        pi = -1;
-        if (e < Y.colptr[Y.lvl_ptr[l + 1]]) {
-            pi = Y.rowind[e];
-            pB = load_b4(blk, U0 + e);
-        }
-        if (tid < Y.lvl_ptr[l + 1] - s) pD = load_b4(blk, Y.di0 + s + tid);
-    };
-    if (nl > 0) fetch_b(nl - 1);
+    // ---- backward
+    if (nl > 0) ahead(nl - 1, Y.colptr, Y.rowind, U0, true);
     for (int l = nl - 1; l >= 0; l--) {
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        int ck[RA];
+        B4<T> cB[RA];
+#pragma unroll
+        for (int r = 0; r < RA; r++) {
+            ck[r] = pk[r];
+            cB[r] = pB[r];
+        }
+        const B4<T> myD = pD;   // narrow: the inverse pivot of node s + grp (lane 0); wide: of node s + tid
+        if (c <= kSweepGroupLevel) {
+            if (l > 0) ahead(l - 1, Y.colptr, Y.rowind, U0, true);
+            if (grp < c) {
+                const int j = s + grp;
+                double a0 = 0, a1 = 0;
+#pragma unroll
+                for (int r = 0; r < RA; r++)
+                    if (ck[r] >= 0) {
+                        const double x0 = x[2 * ck[r]], x1 = x[2 * ck[r] + 1];
+                        a0 += (double)cB[r].a * x0 + (double)cB[r].b * x1;
+                        a1 += (double)cB[r].c * x0 + (double)cB[r].d * x1;
+                    }
+                for (int e = Y.colptr[j] + lane + GL * RA; e < Y.colptr[j + 1]; e += GL) {
+                    const B4<T> U = load_b4(blk, U0 + e);
+                    const int i = Y.rowind[e];
+                    const double x0 = x[2 * i], x1 = x[2 * i + 1];
+                    a0 += (double)U.a * x0 + (double)U.b * x1;
+                    a1 += (double)U.c * x0 + (double)U.d * x1;
+                }
+#pragma unroll
+                for (int off = GL / 2; off > 0; off >>= 1) {
+                    a0 += __shfl_xor(a0, off, GL);
+                    a1 += __shfl_xor(a1, off, GL);
+                }
+                if (lane == 0) {
+                    const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
+                    x[2 * j] = (double)myD.a * y0 + (double)myD.b * y1;
+                    x[2 * j + 1] = (double)myD.c * y0 + (double)myD.d * y1;
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         const int e_lo = Y.colptr[s], e_hi = Y.colptr[s + c];
-        const B4<T> myD = pD;   // inverse pivot of node s + tid
         if (e_lo == e_hi) {     // no entries: only the inverse pivots
-            if (l > 0) fetch_b(l - 1);
+            if (l > 0) ahead(l - 1, Y.colptr, Y.rowind, U0, true);
             for (int i = tid; i < c; i += bd) {
                 const int j = s + i;
                 const B4<T> D = (i == tid) ? myD : load_b4(blk, Y.di0 + j);
@@ -520,8 +616,8 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 int r;
                 B4<T> U;
                 if (e == e_lo + tid) {
-                    r = pi;
-                    U = pB;
+                    r = ck[0];
+                    U = cB[0];
                 } else {
                     r = Y.rowind[e];
                     U = load_b4(blk, U0 + e);
@@ -529,7 +625,7 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 const double x0 = x[2 * r], x1 = x[2 * r + 1];
                 prod[e - pg] = make_double2((double)U.a * x0 + (double)U.b * x1, (double)U.c * x0 + (double)U.d * x1);
             }
-            if (pg == e_lo && l > 0) fetch_b(l - 1);
+            if (pg == e_lo && l > 0) ahead(l - 1, Y.colptr, Y.rowind, U0, true);
             __syncthreads();
             const bool last = pe == e_hi;
             for (int i = tid; i < c; i += bd) {

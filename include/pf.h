@@ -85,9 +85,11 @@ typedef struct pf_options {
     int use_graph;          /* 1: one captured hipGraph per NR iteration, no per-iteration host sync */
     int lu_blocked;         /* sparse LU schedule.  1 (default): the subtrees below the narrow
                                top of the elimination tree in one launch (a workgroup per group
-                               of subtrees) and its trailing dense chain as one dense block
-                               (Gauss-Jordan inverse of its Schur complement); 0: one launch
-                               per elimination-tree level, node by node                        */
+                               of subtrees), the narrow top in one workgroup, both as flat lists
+                               of update pairs level by level, and the tree's trailing dense
+                               chain as one dense block (Gauss-Jordan inverse of its Schur
+                               complement); 0: one launch per elimination-tree level, node by
+                               node, and the narrow top in one workgroup                       */
 } pf_options;
 
 const char *pf_status_str(pf_status s);
@@ -113,7 +115,8 @@ enum pf_bus_field {
     PF_BUS_PG, PF_BUS_QG, PF_BUS_VBASE, PF_BUS_GSH, PF_BUS_BSH, PF_BUS_NFIELDS
 };
 
-/* CDF (.cdf) or CDFX (.cdfx) by extension; v0/theta0 = flat-start values.    */
+/* By extension (CPU/leitura/arquivo.cpp:3-51): CDF (.cdf), CDFX (.cdfx),
+ * MATPOWER (.m) or ANAREDE (.pwf); v0/theta0 = flat-start values.           */
 pf_status pf_read_case(const char *path, double v0, double theta0, pf_network **out);
 void pf_network_free(pf_network *net);
 pf_status pf_network_get_info(const pf_network *net, pf_network_info *info);

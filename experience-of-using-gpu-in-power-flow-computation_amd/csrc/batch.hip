@@ -1498,7 +1498,13 @@ pf_status pf_batch::launch_all() {
     k_zero_int<<<1, 64, 0, st[0]>>>(d_counter, (int)nets.size());
     BHIP(hipEventRecord(ev_mid, st[0]));
     BHIP(hipStreamWaitEvent(st[1], ev_mid, 0));
-    for (size_t i = 0; i < nets.size(); i++) BCHK(launch_net(i, st[i & 1]));
+    // the largest topology first: its few long-running workgroups take their
+    // CUs before the small topology's many short ones fill the rest, so the
+    // two launches overlap instead of the long one waiting for a full GPU
+    std::vector<size_t> order(nets.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return nets[a]->net.nB > nets[c]->net.nB; });
+    for (size_t r = 0; r < order.size(); r++) BCHK(launch_net(order[r], st[r & 1]));
     BHIP(hipEventRecord(ev_join, st[1]));
     BHIP(hipStreamWaitEvent(st[0], ev_join, 0));
     return PF_OK;

@@ -101,6 +101,7 @@ def lib():
         L.pf_probe_free.argtypes = [P]
         L.pf_probe_get.argtypes = [P, P, P, P, P, P]
         L.pf_probe_factor_solve.argtypes = [P, P, P]
+        L.pf_probe_flat_factor_solve.argtypes = [P, P, P]
         L.pf_probe_levels.argtypes = [P, P, P]
         L.pf_probe_chain.argtypes = [P, P, P]
         L.pf_probe_parts.argtypes = [P, P, P, P, P, P]
@@ -236,6 +237,14 @@ class SymbolicProbe:
         b = np.ascontiguousarray(blocks, np.float64).copy()
         x = np.ascontiguousarray(rhs, np.float64).copy()
         _chk(lib().pf_probe_factor_solve(self.h, _p(b), _p(x)), "pf_probe_factor_solve")
+        return x
+
+    def flat_factor_solve(self, blocks: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+        """The same through the order of the GPU's blocked program (flat pair
+        programs, the narrow top's split sweeps), restated on the host."""
+        b = np.ascontiguousarray(blocks, np.float64).copy()
+        x = np.ascontiguousarray(rhs, np.float64).copy()
+        _chk(lib().pf_probe_flat_factor_solve(self.h, _p(b), _p(x)), "pf_probe_flat_factor_solve")
         return x
 
     def __del__(self):

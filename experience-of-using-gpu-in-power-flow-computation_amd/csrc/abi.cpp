@@ -187,4 +187,19 @@ pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, doub
     return PF_OK;
 }
 
+// the same through the order of the GPU's blocked program, restated on the host
+pf_status pf_probe_flat_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x) {
+    if (!p || !blocks || !x) return PF_INVALID_ARG;
+    const pf::Symbolic &s = p->sym;
+    std::vector<double> b(4 * (size_t)s.nblk(), 0.0);
+    std::memcpy(b.data(), blocks, sizeof(double) * 4 * (size_t)s.nslot());
+    pf::FlatTail parts, top;
+    if (s.n_parts > 0) parts = pf::flat_parts(s);
+    if (s.tail_start < s.nlev_s) top = pf::flat_tail(s, s.tail_start, true);
+    if (!pf::host_flat_factor(s, parts, top, b.data())) return PF_SINGULAR;
+    pf::host_split_solve(s, pf::tail_sweeps(s), b.data(), x);
+    std::memcpy(blocks, b.data(), sizeof(double) * 4 * (size_t)s.nslot());
+    return PF_OK;
+}
+
 }  // extern "C"

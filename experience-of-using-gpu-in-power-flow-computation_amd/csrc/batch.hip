@@ -653,28 +653,44 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
 // I = Y E and S = E conj(I): 4 lanes per row
 template <typename T>
 __device__ void wg_pq(const DevNet &N, Work<T> &w) {
+    // two rows a lane group at a time (r and r + rows_per), their entries
+    // walked together so that their loads are in flight together; each row
+    // sums in the same order as alone
     const int rows_per = (int)blockDim.x / 4;
     const int lane = (int)threadIdx.x & 3;
-    for (int r0 = 0; r0 < N.nB; r0 += rows_per) {
-        const int row = r0 + (int)threadIdx.x / 4;
-        double ar = 0, ai = 0;
-        if (row < N.nB) {
-            for (int e = N.rowptr[row] + lane; e < N.rowptr[row + 1]; e += 4) {
-                const double2 Em = w.E[N.colind[e]];
-                const double2 y = N.y[e];
-                ar += y.x * Em.x - y.y * Em.y;
-                ai += y.x * Em.y + y.y * Em.x;
-            }
+    for (int r0 = 0; r0 < N.nB; r0 += 2 * rows_per) {
+        int row[2], e[2], e1[2];
+        double ar[2] = {0, 0}, ai[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            row[u] = r0 + u * rows_per + (int)threadIdx.x / 4;
+            const bool ok = row[u] < N.nB;
+            e[u] = ok ? N.rowptr[row[u]] + lane : 0;
+            e1[u] = ok ? N.rowptr[row[u] + 1] : 0;
         }
-        ar += __shfl_xor(ar, 1, 4);
-        ai += __shfl_xor(ai, 1, 4);
-        ar += __shfl_xor(ar, 2, 4);
-        ai += __shfl_xor(ai, 2, 4);
-        if (row < N.nB && lane == 0) {
-            const double2 Ek = w.E[row];
-            w.I[row] = make_double2(ar, ai);
-            w.P[row] = Ek.x * ar + Ek.y * ai;
-            w.Q[row] = Ek.y * ar - Ek.x * ai;
+        while (e[0] < e1[0] || e[1] < e1[1]) {
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+                if (e[u] < e1[u]) {
+                    const double2 Em = w.E[N.colind[e[u]]];
+                    const double2 y = N.y[e[u]];
+                    ar[u] += y.x * Em.x - y.y * Em.y;
+                    ai[u] += y.x * Em.y + y.y * Em.x;
+                    e[u] += 4;
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            ar[u] += __shfl_xor(ar[u], 1, 4);
+            ai[u] += __shfl_xor(ai[u], 1, 4);
+            ar[u] += __shfl_xor(ar[u], 2, 4);
+            ai[u] += __shfl_xor(ai[u], 2, 4);
+            if (row[u] < N.nB && lane == 0) {
+                const double2 Ek = w.E[row[u]];
+                w.I[row[u]] = make_double2(ar[u], ai[u]);
+                w.P[row[u]] = Ek.x * ar[u] + Ek.y * ai[u];
+                w.Q[row[u]] = Ek.y * ar[u] - Ek.x * ai[u];
+            }
         }
     }
 }
@@ -682,43 +698,58 @@ __device__ void wg_pq(const DevNet &N, Work<T> &w) {
 // r = g - J x (FP64, complex voltages)
 template <typename T>
 __device__ void wg_residual(const DevNet &N, Work<T> &w) {
+    // two rows a lane group at a time, as in wg_pq
     const int rows_per = (int)blockDim.x / 4;
     const int lane = (int)threadIdx.x & 3;
-    for (int r0 = 0; r0 < N.nB; r0 += rows_per) {
-        const int row = r0 + (int)threadIdx.x / 4;
-        const bool active = row < N.nB && row != N.slack;
-        double ar = 0, ai = 0;
-        if (active) {
-            for (int e = N.rowptr[row] + lane; e < N.rowptr[row + 1]; e += 4) {
-                const int m = N.colind[e];
-                if (m == N.slack) continue;
-                const int jm = N.node_of_bus[m];
-                const double dth = w.x[2 * jm];
-                const double dv = w.isq[m] ? w.x[2 * jm + 1] : 0.0;
-                const double2 Em = w.E[m];
-                const double f = dv / w.V[m];
-                const double dr = Em.x * f - Em.y * dth, di = Em.y * f + Em.x * dth;
-                const double2 y = N.y[e];
-                ar += y.x * dr - y.y * di;
-                ai += y.x * di + y.y * dr;
-            }
+    for (int r0 = 0; r0 < N.nB; r0 += 2 * rows_per) {
+        int row[2], e[2], e1[2];
+        bool active[2];
+        double ar[2] = {0, 0}, ai[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            row[u] = r0 + u * rows_per + (int)threadIdx.x / 4;
+            active[u] = row[u] < N.nB && row[u] != N.slack;
+            e[u] = active[u] ? N.rowptr[row[u]] + lane : 0;
+            e1[u] = active[u] ? N.rowptr[row[u] + 1] : 0;
         }
-        ar += __shfl_xor(ar, 1, 4);
-        ai += __shfl_xor(ai, 1, 4);
-        ar += __shfl_xor(ar, 2, 4);
-        ai += __shfl_xor(ai, 2, 4);
-        if (active && lane == 0) {
-            const int j = N.node_of_bus[row];
-            const double dth = w.x[2 * j];
-            const bool q = w.isq[row];
-            const double dv = q ? w.x[2 * j + 1] : 0.0;
-            const double2 Ek = w.E[row], Ik = w.I[row];
-            const double f = dv / w.V[row];
-            const double dr = Ek.x * f - Ek.y * dth, di = Ek.y * f + Ek.x * dth;
-            const double sr = dr * Ik.x + di * Ik.y + Ek.x * ar + Ek.y * ai;
-            const double si = di * Ik.x - dr * Ik.y + Ek.y * ar - Ek.x * ai;
-            w.r[2 * j] = w.g[2 * j] - sr;
-            w.r[2 * j + 1] = q ? w.g[2 * j + 1] - si : -w.x[2 * j + 1];
+        while (e[0] < e1[0] || e[1] < e1[1]) {
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+                if (e[u] < e1[u]) {
+                    const int m = N.colind[e[u]];
+                    if (m != N.slack) {
+                        const int jm = N.node_of_bus[m];
+                        const double dth = w.x[2 * jm];
+                        const double dv = w.isq[m] ? w.x[2 * jm + 1] : 0.0;
+                        const double2 Em = w.E[m];
+                        const double f = dv / w.V[m];
+                        const double dr = Em.x * f - Em.y * dth, di = Em.y * f + Em.x * dth;
+                        const double2 y = N.y[e[u]];
+                        ar[u] += y.x * dr - y.y * di;
+                        ai[u] += y.x * di + y.y * dr;
+                    }
+                    e[u] += 4;
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            ar[u] += __shfl_xor(ar[u], 1, 4);
+            ai[u] += __shfl_xor(ai[u], 1, 4);
+            ar[u] += __shfl_xor(ar[u], 2, 4);
+            ai[u] += __shfl_xor(ai[u], 2, 4);
+            if (active[u] && lane == 0) {
+                const int k = row[u], j = N.node_of_bus[k];
+                const double dth = w.x[2 * j];
+                const bool q = w.isq[k];
+                const double dv = q ? w.x[2 * j + 1] : 0.0;
+                const double2 Ek = w.E[k], Ik = w.I[k];
+                const double f = dv / w.V[k];
+                const double dr = Ek.x * f - Ek.y * dth, di = Ek.y * f + Ek.x * dth;
+                const double sr = dr * Ik.x + di * Ik.y + Ek.x * ar[u] + Ek.y * ai[u];
+                const double si = di * Ik.x - dr * Ik.y + Ek.y * ar[u] - Ek.x * ai[u];
+                w.r[2 * j] = w.g[2 * j] - sr;
+                w.r[2 * j + 1] = q ? w.g[2 * j + 1] - si : -w.x[2 * j + 1];
+            }
         }
     }
 }
@@ -847,13 +878,33 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
             break;
         }
         if (!(maxG < J.diverge)) break;                   // NaN or diverging: give up this start
-        // Jacobian blocks
-        for (int s = tid; s < tp.nslot; s += blockDim.x) {
-            const int e = Y.slot_e[s];
+        // Jacobian blocks, four slots a thread at a time so that their index
+        // loads are in flight together
+        constexpr int JF = 4;
+        for (int s0 = tid; s0 < tp.nslot; s0 += JF * blockDim.x) {
+            int es[JF];
+#pragma unroll
+            for (int r = 0; r < JF; r++) {
+                const int s = s0 + r * (int)blockDim.x;
+                es[r] = s < tp.nslot ? Y.slot_e[s] : -1;
+            }
+            int ks[JF], ms[JF];
+            double2 ys[JF];
+#pragma unroll
+            for (int r = 0; r < JF; r++) {
+                const int e = es[r] >= 0 ? es[r] : 0;
+                ks[r] = N.coorow[e];
+                ms[r] = N.colind[e];
+                ys[r] = N.y[e];
+            }
+#pragma unroll
+            for (int r = 0; r < JF; r++) {
+            const int s = s0 + r * (int)blockDim.x, e = es[r];
+            if (s >= tp.nslot) break;
             B4<T> out{0, 0, 0, 0};
             if (e >= 0) {
-                const int k = N.coorow[e], m = N.colind[e];
-                const double2 y = N.y[e];
+                const int k = ks[r], m = ms[r];
+                const double2 y = ys[r];
                 double H, Nn, M, L;
                 if (k != m) {
                     const double2 Em = w.E[m], Ek = w.E[k];
@@ -870,6 +921,7 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
                 out = B4<T>{(T)H, (T)Nn, (T)M, (T)L};
             }
             store_b4(w.blk, s, out);
+            }
         }
         if (tid == 0) s_maxDx_bits = 0;
         __syncthreads();

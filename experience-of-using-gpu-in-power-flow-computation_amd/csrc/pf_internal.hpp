@@ -185,6 +185,15 @@ std::vector<int> min_degree(int n, const std::vector<int> &rowptr, const std::ve
 bool host_block_factor(const Symbolic &s, double *blocks);
 void host_block_solve(const Symbolic &s, const double *blocks, double *x /* 2n, node order */);
 
+// The same factor and solve in the order of the device's blocked program:
+// the subtree groups' flat steps (parts: flat_parts, or empty), the narrow
+// top's pairs from below it and its flat steps (top: flat_tail with
+// split_below), the dense chain; the solve with the narrow top's sweep
+// split (tail_sweeps).  Same layout and results as host_block_factor /
+// host_block_solve up to the order of the sums.
+bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &top, double *blocks);
+void host_split_solve(const Symbolic &s, const TailSweeps &ts, const double *blocks, double *x);
+
 // text of the last failure on this thread, read back by pf_last_error()
 void set_last_error(const std::string &what);
 

@@ -630,6 +630,153 @@ bool host_block_factor(const Symbolic &s, double *blk) {
     return true;
 }
 
+bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &top, double *blk) {
+    const int n = s.n, L0 = n, U0 = n + s.nnzL;
+    const int nf = s.ch_m > 0 ? s.ch_c0 : n;
+    std::vector<double> inv(4 * (size_t)n, 0.0);   // inverse pivots; the pivot slots keep A_jj
+    auto L = [&](int p) { return blk + 4 * (size_t)(L0 + p); };
+    auto U = [&](int p) { return blk + 4 * (size_t)(U0 + p); };
+    auto sum = [](const std::vector<double> &v, int lo, int hi, double *a) {
+        a[0] = a[1] = a[2] = a[3] = 0.0;
+        for (int q = lo; q < hi; q++)
+            for (int c = 0; c < 4; c++) a[c] += v[4 * (size_t)q + c];
+    };
+    auto run = [&](const FlatTail &f) -> bool {
+        const int nsteps = (int)f.fl_ptr.size() - 1;
+        for (int t = 0; t < nsteps; t++) {
+            const int base = f.fl_ptr[t], np = f.fl_ptr[t + 1] - base, ns = f.fl_split[t];
+            std::vector<double> pa(4 * (size_t)np), pu(4 * (size_t)std::max(np - ns, 0));
+            for (int e = 0; e < np; e++) {   // every product from earlier steps' blocks
+                mul2(&pa[4 * (size_t)e], L(f.fl_a[base + e]), U(f.fl_b[base + e]));
+                if (e >= ns) mul2(&pu[4 * (size_t)(e - ns)], L(f.fl_b[base + e]), U(f.fl_a[base + e]));
+            }
+            for (int E = f.de_ptr[t]; E < f.de_ptr[t + 1]; E++) {
+                const int j = f.de_node[E], p = f.de_p[E];
+                double a[4], P[4];
+                sum(pa, f.de_lo[E], f.de_hi[E], a);
+                double ad[4];
+                if (p < 0) std::copy(a, a + 4, ad);
+                else sum(pa, f.de_dlo[E], f.de_dhi[E], ad);
+                const double *D = blk + 4 * (size_t)j;
+                for (int c = 0; c < 4; c++) P[c] = D[c] - ad[c];
+                double det = P[0] * P[3] - P[1] * P[2];
+                if (!(det != 0.0) || !std::isfinite(det)) return false;
+                const double Pi[4] = {P[3] / det, -P[1] / det, -P[2] / det, P[0] / det};
+                if (p < 0) {
+                    std::copy(Pi, Pi + 4, &inv[4 * (size_t)j]);
+                } else {
+                    double u[4], Lb[4], lb[4];
+                    sum(pu, f.de_lo[E] - ns, f.de_hi[E] - ns, u);
+                    for (int c = 0; c < 4; c++) {
+                        Lb[c] = L(p)[c] - a[c];
+                        U(p)[c] -= u[c];
+                    }
+                    mul2(lb, Lb, Pi);
+                    std::copy(lb, lb + 4, L(p));
+                }
+            }
+        }
+        return true;
+    };
+    if (!parts.fl_ptr.empty() && !run(parts)) return false;
+    // the narrow top's pairs from below it, applied first
+    for (size_t E = 0; E < top.de_node.size(); E++) {
+        const int j = top.de_node[E], p = top.de_p[E];
+        double a[4] = {0, 0, 0, 0}, u[4] = {0, 0, 0, 0}, t4[4];
+        for (int q = top.de_q0[E]; q < top.de_ext[E]; q++) {
+            mul2(t4, L(s.fc_a[q]), U(s.fc_b[q]));
+            for (int c = 0; c < 4; c++) a[c] += t4[c];
+            if (p >= 0) {
+                mul2(t4, L(s.fc_b[q]), U(s.fc_a[q]));
+                for (int c = 0; c < 4; c++) u[c] += t4[c];
+            }
+        }
+        double *v0 = p < 0 ? blk + 4 * (size_t)j : L(p);
+        for (int c = 0; c < 4; c++) v0[c] -= a[c];
+        if (p >= 0)
+            for (int c = 0; c < 4; c++) U(p)[c] -= u[c];
+    }
+    if (!top.fl_ptr.empty() && !run(top)) return false;
+    for (int j = 0; j < nf; j++) std::copy(&inv[4 * (size_t)j], &inv[4 * (size_t)j] + 4, blk + 4 * (size_t)j);
+    if (s.ch_m > 0) return host_chain_factor(s, blk);
+    return true;
+}
+
+void host_split_solve(const Symbolic &s, const TailSweeps &ts, const double *blk, double *x) {
+    const int n = s.n, L0 = n, U0 = n + s.nnzL, t0 = ts.t0, t1 = ts.t1;
+    auto Lr = [&](int q) { return blk + 4 * (size_t)(L0 + s.rs_p[q]); };   // L by row position q
+    auto sub = [&](double *y, const double *B, int k) {
+        y[0] -= B[0] * x[2 * k] + B[1] * x[2 * k + 1];
+        y[1] -= B[2] * x[2 * k] + B[3] * x[2 * k + 1];
+    };
+    for (int j = 0; j < t0; j++) {   // below the top: every row entry
+        double y[2] = {x[2 * j], x[2 * j + 1]};
+        for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) sub(y, Lr(q), s.rs_k[q]);
+        x[2 * j] = y[0];
+        x[2 * j + 1] = y[1];
+    }
+    for (int a = 0; a < t1 - t0; a++) {   // the top: entries below it first
+        double y[2] = {x[2 * (t0 + a)], x[2 * (t0 + a) + 1]};
+        for (int e = ts.fx_ptr[a]; e < ts.fx_ptr[a + 1]; e++) sub(y, Lr(ts.fx_q[e]), ts.fx_k[e]);
+        x[2 * (t0 + a)] = y[0];
+        x[2 * (t0 + a) + 1] = y[1];
+    }
+    for (int a = 0; a < t1 - t0; a++) {   // then its own entries, in node (level) order
+        double y[2] = {x[2 * (t0 + a)], x[2 * (t0 + a) + 1]};
+        for (int e = ts.fi_ptr[a]; e < ts.fi_ptr[a + 1]; e++) sub(y, Lr(ts.fi_q[e]), t0 + ts.fi_k[e]);
+        x[2 * (t0 + a)] = y[0];
+        x[2 * (t0 + a) + 1] = y[1];
+    }
+    if (s.ch_m > 0) {   // x_C = S^-1 (x_C - L_CN y_N)
+        const int m = s.ch_m, c0 = s.ch_c0;
+        std::vector<double> r(2 * (size_t)m);
+        for (int a = 0; a < m; a++) {
+            const int c = c0 + a;
+            double y[2] = {x[2 * c], x[2 * c + 1]};
+            for (int q = s.rs_ptr[c]; q < s.rs_ptr[c + 1] && s.rs_k[q] < c0; q++) sub(y, Lr(q), s.rs_k[q]);
+            r[2 * a] = y[0];
+            r[2 * a + 1] = y[1];
+        }
+        const double *S = blk + 4 * (size_t)(n + 3 * s.nnzL);
+        for (int a = 0; a < m; a++) {
+            double a0 = 0, a1 = 0;
+            for (int b = 0; b < m; b++) {
+                const double *Sab = S + 4 * ((size_t)a * m + b);
+                a0 += Sab[0] * r[2 * b] + Sab[1] * r[2 * b + 1];
+                a1 += Sab[2] * r[2 * b] + Sab[3] * r[2 * b + 1];
+            }
+            x[2 * (c0 + a)] = a0;
+            x[2 * (c0 + a) + 1] = a1;
+        }
+    }
+    std::vector<double> y(2 * (size_t)(t1 - t0));
+    for (int a = 0; a < t1 - t0; a++) {   // the top backward: entries on the dense chain first
+        y[2 * a] = x[2 * (t0 + a)];
+        y[2 * a + 1] = x[2 * (t0 + a) + 1];
+        for (int e = ts.bx_ptr[a]; e < ts.bx_ptr[a + 1]; e++) {
+            double v[2] = {y[2 * a], y[2 * a + 1]};
+            sub(v, blk + 4 * (size_t)(U0 + ts.bx_p[e]), ts.bx_i[e]);
+            y[2 * a] = v[0];
+            y[2 * a + 1] = v[1];
+        }
+    }
+    for (int a = t1 - t0 - 1; a >= 0; a--) {   // then its own, in reverse node order
+        const int j = t0 + a;
+        double v[2] = {y[2 * a], y[2 * a + 1]};
+        for (int e = ts.bi_ptr[a]; e < ts.bi_ptr[a + 1]; e++) sub(v, blk + 4 * (size_t)(U0 + ts.bi_p[e]), t0 + ts.bi_i[e]);
+        const double *Di = blk + 4 * (size_t)j;
+        x[2 * j] = Di[0] * v[0] + Di[1] * v[1];
+        x[2 * j + 1] = Di[2] * v[0] + Di[3] * v[1];
+    }
+    for (int j = t0 - 1; j >= 0; j--) {
+        double v[2] = {x[2 * j], x[2 * j + 1]};
+        for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) sub(v, blk + 4 * (size_t)(U0 + p), s.rowind[p]);
+        const double *Di = blk + 4 * (size_t)j;
+        x[2 * j] = Di[0] * v[0] + Di[1] * v[1];
+        x[2 * j + 1] = Di[2] * v[0] + Di[3] * v[1];
+    }
+}
+
 void host_block_solve(const Symbolic &s, const double *blk, double *x) {
     const int n = s.n, L0 = n, U0 = n + s.nnzL;
     const int nf = s.ch_m > 0 ? s.ch_c0 : n;   // chain rows take only x_C - L_CN y_N here

@@ -248,6 +248,10 @@ pf_status pf_probe_parts(const pf_symbolic_probe *p, int *n_parts, int *part_lev
                          int *part_nodes, int *parent);
 /* blocks: nslot 2x2 blocks (row-major), x: 2n right-hand side in node order */
 pf_status pf_probe_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x);
+/* The same factor and solve in the order of the GPU's blocked program (the
+ * subtree groups' and the narrow top's flat pair programs, the narrow top's
+ * split sweeps), restated on the host.                                      */
+pf_status pf_probe_flat_factor_solve(const pf_symbolic_probe *p, double *blocks, double *x);
 
 /* ---------------------------------------------------------------- batches */
 /* Many scenarios of one or more topologies solved in one persistent launch:

@@ -321,3 +321,45 @@ def test_batch_1888(pf, oracle, synth, cases, precision):
             assert row[7] <= 1e-6 and row[8] <= 1e-6, row
         if precision == "fp64":
             assert row[3] == row[4] and row[5] == row[6], row
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_odd_size_network(pf, oracle, synth, precision):
+    """A network with an odd bus count (IEEE 57-bus): the per-scenario working
+    set's complex arrays sit after 5 nB doubles, so they must be realigned to
+    16 bytes.  Every placement (LDS, step vectors in LDS, all global) solves
+    each scenario like the oracle.  The two placements in global memory are
+    refused for a working set that fits in LDS, as this one does, so the
+    LDS placement carries the check; all placements share the layout code
+    (carve)."""
+    p57 = os.path.join(GOLDEN, "ieee", "ieee57.cdf")
+    n57 = pf.Network(p57)
+    assert n57.nB % 2 == 1
+    o57 = oracle.Case(p57)
+    P, Q, L = _scenarios(synth, n57, 6, 3600000)
+    ref = _oracle_solve(o57, P, Q, L, lim=True)
+    opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1,
+                      precision=pf.MIXED if precision == "mixed" else pf.FP64)
+    b = pf.Batch([n57], opts)
+    b.set_scenarios(np.zeros(6, np.int32), P.ravel(), Q.ravel(), L.ravel())
+    ran = []
+    for mode in (2, 1, 0):
+        try:
+            b.set_shape(0, mode, 256)
+        except pf.PfError:
+            continue
+        ran.append(mode)
+        _say("ieee57", precision, "mode", mode)
+        b.solve()
+        V, th, it, cv, mm = b.results()
+        lq, _ = b.limq()
+        V, th = V.reshape(6, -1), th.reshape(6, -1)
+        for s, r in enumerate(ref):
+            assert bool(cv[s]) == bool(r["converged"]), (mode, s)
+            assert np.array_equal(lq[s], r["limQ"]), (mode, s)
+            if r["converged"]:
+                assert np.abs(V[s] - r["V"]).max() <= 1e-6, (mode, s)
+                assert np.abs(th[s] - r["theta"]).max() <= 1e-6, (mode, s)
+                if precision == "fp64":
+                    assert it[s] == r["iterations"], (mode, s)
+    assert 2 in ran

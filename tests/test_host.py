@@ -175,6 +175,11 @@ def test_symbolic_program_solves(pf, cases, which):
     x = pr.factor_solve(blocks, b)
     xr = np.linalg.solve(_dense_from_slots(net, pr, blocks), b)
     assert np.abs(x - xr).max() <= 1e-12 * np.abs(xr).max()
+    # the GPU's blocked order: the subtree groups' and the narrow top's flat
+    # pair programs (the top's pairs from below it applied first) and the
+    # top's split sweeps give the same solution
+    xf = pr.flat_factor_solve(blocks, b)
+    assert np.abs(xf - xr).max() <= 1e-12 * np.abs(xr).max()
     # levels are contiguous node ranges, and the chain is the last levels
     lp = pr.lvl_ptr
     assert lp[0] == 0 and lp[-1] == pr.n and np.all(np.diff(lp) > 0)

@@ -197,7 +197,9 @@ pf_status pf_probe_flat_factor_solve(const pf_symbolic_probe *p, double *blocks,
     if (s.n_parts > 0) parts = pf::flat_parts(s);
     if (s.tail_start < s.nlev_s) top = pf::flat_tail(s, s.tail_start, true);
     if (!pf::host_flat_factor(s, parts, top, b.data())) return PF_SINGULAR;
-    pf::host_split_solve(s, pf::tail_sweeps(s), b.data(), x);
+    pf::PartSweeps ps;
+    if (s.n_parts > 0) ps = pf::part_sweeps(s);
+    pf::host_split_solve(s, pf::tail_sweeps(s), ps, b.data(), x);
     std::memcpy(blocks, b.data(), sizeof(double) * 4 * (size_t)s.nslot());
     return PF_OK;
 }

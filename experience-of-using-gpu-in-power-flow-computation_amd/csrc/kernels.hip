@@ -816,6 +816,117 @@ __global__ void __launch_bounds__(1024) k_bwd_tail_lds(DevSym Y, DevTail D, cons
     }
 }
 
+// ----------------------------------- subtree groups' sweeps with x in LDS
+// Workgroup g holds its group's unknowns, its entries' local indices and
+// factor blocks in LDS for the whole sweep (a group's forward rows only
+// reach its own nodes; the backward columns' entries above the group are
+// final before its sweep and are subtracted first).  16 lanes per node, one
+// barrier per level.  LDS: x [max_nodes], then (backward: the inverse
+// pivots [max_nodes]), the entry blocks and their local indices.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_parts_lds(DevSym Y, DevParts P, const T *blk, double *x, Ctl *ctl,
+                                                       int gate) {
+    if (gate && !ctl->step) return;
+    extern __shared__ __align__(16) char smem[];
+    const int T_ = Y.part_levels, g = (int)blockIdx.x;
+    const int u0 = Y.part_ptr[(size_t)g * T_], u1 = Y.part_ptr[(size_t)(g + 1) * T_], nn = u1 - u0;
+    const int e0 = P.fi_ptr[u0], ne = P.fi_ptr[u1] - e0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / 16, lane = tid & 15, ng = bd / 16;
+    const int LR0 = Y.n + 2 * Y.nnzL;
+    double2 *xl = reinterpret_cast<double2 *>(smem);
+    T *eb = reinterpret_cast<T *>(xl + P.max_nodes);
+    int *ek = reinterpret_cast<int *>(eb + 4 * (size_t)P.max_fi);
+    for (int a = tid; a < nn; a += bd) {
+        const int j = Y.part_nodes[u0 + a];
+        xl[a] = make_double2(x[2 * j], x[2 * j + 1]);
+    }
+    for (int e = tid; e < ne; e += bd) {
+        ek[e] = P.fi_k[e0 + e];
+        store_b4(eb, e, load_b4(blk, LR0 + P.fi_q[e0 + e]));
+    }
+    __syncthreads();
+    for (int l = 0; l < T_; l++) {
+        const int a0 = Y.part_ptr[(size_t)g * T_ + l] - u0, a1 = Y.part_ptr[(size_t)g * T_ + l + 1] - u0;
+        for (int a = a0 + grp; a < a1; a += ng) {
+            double s0 = 0, s1 = 0;
+            for (int e = P.fi_ptr[u0 + a] - e0 + lane; e < P.fi_ptr[u0 + a + 1] - e0; e += 16) {
+                const B4<T> L = load_b4(eb, e);
+                const double2 xk = xl[ek[e]];
+                s0 += (double)L.a * xk.x + (double)L.b * xk.y;
+                s1 += (double)L.c * xk.x + (double)L.d * xk.y;
+            }
+            sum16(s0, s1);
+            if (lane == 0) xl[a] = make_double2(xl[a].x - s0, xl[a].y - s1);
+        }
+        __syncthreads();
+    }
+    for (int a = tid; a < nn; a += bd) {
+        const int j = Y.part_nodes[u0 + a];
+        x[2 * j] = xl[a].x;
+        x[2 * j + 1] = xl[a].y;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_parts_lds(DevSym Y, DevParts P, const T *blk, double *x, Ctl *ctl,
+                                                       int gate) {
+    if (gate && !ctl->step) return;
+    extern __shared__ __align__(16) char smem[];
+    const int T_ = Y.part_levels, g = (int)blockIdx.x;
+    const int u0 = Y.part_ptr[(size_t)g * T_], u1 = Y.part_ptr[(size_t)(g + 1) * T_], nn = u1 - u0;
+    const int e0 = P.bi_ptr[u0], ne = P.bi_ptr[u1] - e0;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / 16, lane = tid & 15, ng = bd / 16;
+    const int U0 = Y.n + Y.nnzL;
+    double2 *xl = reinterpret_cast<double2 *>(smem);
+    T *dg = reinterpret_cast<T *>(xl + P.max_nodes);
+    T *eb = dg + 4 * (size_t)P.max_nodes;
+    int *ek = reinterpret_cast<int *>(eb + 4 * (size_t)P.max_bi);
+    // the entries above the group first (their x are final; a node has few):
+    // a thread per node, with its inverse pivot
+    for (int a = tid; a < nn; a += bd) {
+        const int j = Y.part_nodes[u0 + a];
+        double s0 = 0, s1 = 0;
+        for (int e = P.bx_ptr[u0 + a]; e < P.bx_ptr[u0 + a + 1]; e++) {
+            const B4<T> U = load_b4(blk, U0 + P.bx_p[e]);
+            const int i = P.bx_i[e];
+            const double x0 = x[2 * i], x1 = x[2 * i + 1];
+            s0 += (double)U.a * x0 + (double)U.b * x1;
+            s1 += (double)U.c * x0 + (double)U.d * x1;
+        }
+        xl[a] = make_double2(x[2 * j] - s0, x[2 * j + 1] - s1);
+        store_b4(dg, a, load_b4(blk, Y.di0 + j));
+    }
+    for (int e = tid; e < ne; e += bd) {
+        ek[e] = P.bi_i[e0 + e];
+        store_b4(eb, e, load_b4(blk, U0 + P.bi_p[e0 + e]));
+    }
+    __syncthreads();
+    for (int l = T_ - 1; l >= 0; l--) {
+        const int a0 = Y.part_ptr[(size_t)g * T_ + l] - u0, a1 = Y.part_ptr[(size_t)g * T_ + l + 1] - u0;
+        for (int a = a0 + grp; a < a1; a += ng) {
+            double s0 = 0, s1 = 0;
+            for (int e = P.bi_ptr[u0 + a] - e0 + lane; e < P.bi_ptr[u0 + a + 1] - e0; e += 16) {
+                const B4<T> U = load_b4(eb, e);
+                const double2 xi = xl[ek[e]];
+                s0 += (double)U.a * xi.x + (double)U.b * xi.y;
+                s1 += (double)U.c * xi.x + (double)U.d * xi.y;
+            }
+            sum16(s0, s1);
+            if (lane == 0) {
+                const double y0 = xl[a].x - s0, y1 = xl[a].y - s1;
+                const B4<T> D = load_b4(dg, a);
+                xl[a] = make_double2((double)D.a * y0 + (double)D.b * y1, (double)D.c * y0 + (double)D.d * y1);
+            }
+        }
+        __syncthreads();
+    }
+    for (int a = tid; a < nn; a += bd) {
+        const int j = Y.part_nodes[u0 + a];
+        x[2 * j] = xl[a].x;
+        x[2 * j + 1] = xl[a].y;
+    }
+}
+
 // ------------------------------------------------------- branch flows
 // Flows as the reference reports them (SpCalcFlux,
 // CPU/esparso/potencia/spCalcFluxPQ.cpp:3-36): for the direction k -> m,
@@ -1055,6 +1166,10 @@ template __global__ void k_factor_tail_flat<double>(DevSym, DevFlat, double *, C
 template __global__ void k_factor_tail_flat<float>(DevSym, DevFlat, float *, Ctl *, int, int);
 template __global__ void k_factor_parts_flat<double>(DevSym, DevFlat, double *, Ctl *, int);
 template __global__ void k_factor_parts_flat<float>(DevSym, DevFlat, float *, Ctl *, int);
+template __global__ void k_fwd_parts_lds<double>(DevSym, DevParts, const double *, double *, Ctl *, int);
+template __global__ void k_fwd_parts_lds<float>(DevSym, DevParts, const float *, double *, Ctl *, int);
+template __global__ void k_bwd_parts_lds<double>(DevSym, DevParts, const double *, double *, Ctl *, int);
+template __global__ void k_bwd_parts_lds<float>(DevSym, DevParts, const float *, double *, Ctl *, int);
 template __global__ void k_fwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);
 template __global__ void k_fwd_level<float>(DevSym, const float *, double *, Ctl *, int, int, int);
 template __global__ void k_bwd_level<double>(DevSym, const double *, double *, Ctl *, int, int, int);

@@ -123,6 +123,22 @@ struct DevFlat {
     const int *de_node, *de_p, *de_q0, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi;
 };
 
+// The subtree groups' sweeps with their unknowns in LDS (PartSweeps on the
+// host); the group ranges are the DevSym's part_ptr / part_nodes.
+struct DevParts {
+    int max_nodes, max_fi, max_bi;
+    const int *fi_ptr, *fi_k, *fi_q, *bi_ptr, *bi_i, *bi_p, *bx_ptr, *bx_i, *bx_p;
+};
+
+// bytes of dynamic LDS of a subtree group's sweep: its unknowns, its
+// entries' local indices and blocks, and (backward) its inverse pivots
+template <typename T>
+inline long long part_sweep_lds(int max_nodes, int max_entries, bool backward) {
+    long long b = 16LL * max_nodes + (4LL + 4LL * (long long)sizeof(T)) * max_entries;
+    if (backward) b += 4LL * (long long)sizeof(T) * max_nodes;
+    return b;
+}
+
 // bytes of LDS of the flat factorisation: the pair products of one level
 // (the second product only for column destinations)
 template <typename T>

@@ -22,6 +22,8 @@ template <typename T> __global__ void k_chain_schur(DevSym Y, T *blk, Ctl *ctl, 
 template <typename T> __global__ void k_chain_invert(DevSym Y, T *blk, Ctl *ctl, int gate);
 __global__ void k_flows(DevNet N, DevState S, const FlowBranch *br, int nL, double *out);
 template <typename T> __global__ void k_fwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate);
+template <typename T> __global__ void k_fwd_parts_lds(DevSym Y, DevParts P, const T *blk, double *x, Ctl *ctl, int gate);
+template <typename T> __global__ void k_bwd_parts_lds(DevSym Y, DevParts P, const T *blk, double *x, Ctl *ctl, int gate);
 template <typename T> __global__ void k_bwd_parts(DevSym Y, const T *blk, double *x, Ctl *ctl, int gate);
 template <typename T> __global__ void k_fwd_level(DevSym Y, const T *blk, double *x, Ctl *ctl, int l0, int count, int gate);
 template <typename T> __global__ void k_bwd_level(DevSym Y, const T *blk, double *x, Ctl *ctl, int l0, int count, int gate);

@@ -170,6 +170,21 @@ struct FlatTail {
 FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below);
 FlatTail flat_parts(const Symbolic &s);
 
+// The subtree groups' sweeps with each group's unknowns held by its
+// workgroup.  Per node u of part_nodes (group g owns u in [part_ptr[g T],
+// part_ptr[(g+1) T])), in that group's local numbering (u - part_ptr[g T]):
+//   forward  fi[fi_ptr[u], fi_ptr[u+1]): (local k, row-ordered L slot q),
+//            every entry of the row (its sources are the group's own);
+//   backward bi[bi_ptr[u], bi_ptr[u+1]): (local i, L position p) for the
+//            column's entries inside the group, bx[bx_ptr[u], ...): (node i,
+//            p) for those above it (final before the group's sweep).
+// max_nodes / max_fi / max_bi bound one group.
+struct PartSweeps {
+    std::vector<int> fi_ptr, fi_k, fi_q, bi_ptr, bi_i, bi_p, bx_ptr, bx_i, bx_p;
+    int max_nodes = 0, max_fi = 0, max_bi = 0;
+};
+PartSweeps part_sweeps(const Symbolic &s);
+
 // Splits the levels below s.tail_start into at most max_parts groups of
 // whole subtrees, balanced by factorisation work (longest processing time
 // first).
@@ -188,11 +203,13 @@ void host_block_solve(const Symbolic &s, const double *blocks, double *x /* 2n, 
 // The same factor and solve in the order of the device's blocked program:
 // the subtree groups' flat steps (parts: flat_parts, or empty), the narrow
 // top's pairs from below it and its flat steps (top: flat_tail with
-// split_below), the dense chain; the solve with the narrow top's sweep
-// split (tail_sweeps).  Same layout and results as host_block_factor /
-// host_block_solve up to the order of the sums.
+// split_below), the dense chain; the solve with the subtree groups' and the
+// narrow top's sweep splits (part_sweeps, tail_sweeps).  Same layout and
+// results as host_block_factor / host_block_solve up to the order of the
+// sums.
 bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &top, double *blocks);
-void host_split_solve(const Symbolic &s, const TailSweeps &ts, const double *blocks, double *x);
+void host_split_solve(const Symbolic &s, const TailSweeps &ts, const PartSweeps &ps, const double *blocks,
+                      double *x);
 
 // text of the last failure on this thread, read back by pf_last_error()
 void set_last_error(const std::string &what);

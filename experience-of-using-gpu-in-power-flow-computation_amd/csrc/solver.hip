@@ -67,6 +67,9 @@ struct pf_solver {
     // flat tail factorisation (blocked program): per precision [double,
     // float] its dynamic LDS bytes, 0 = not used (the chunked walk)
     DevFlat dfl{}, dfp{};
+    // subtree groups' sweeps with x in LDS: per precision, forward and backward LDS bytes (0: not used)
+    DevParts dps{};
+    int psw_lds[2][2] = {{0, 0}, {0, 0}};
     int flat_ne = 0;
     int flat_lds[2] = {0, 0}, parts_lds[2] = {0, 0};
     int tail_mode[2] = {0, 0};
@@ -471,6 +474,27 @@ pf_status pf_solver::setup_flat_tail() {
     PFCHK(allow((const void *)k_factor_tail_flat<float>,
                 std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16), flat_lds[1]));
     if (sym.n_parts > 0) {
+        const PartSweeps ps = part_sweeps(sym);
+        dps.max_nodes = ps.max_nodes;
+        dps.max_fi = ps.max_fi;
+        dps.max_bi = ps.max_bi;
+        PFCHK(upload((int **)&dps.fi_ptr, ps.fi_ptr));
+        PFCHK(upload((int **)&dps.fi_k, ps.fi_k));
+        PFCHK(upload((int **)&dps.fi_q, ps.fi_q));
+        PFCHK(upload((int **)&dps.bi_ptr, ps.bi_ptr));
+        PFCHK(upload((int **)&dps.bi_i, ps.bi_i));
+        PFCHK(upload((int **)&dps.bi_p, ps.bi_p));
+        PFCHK(upload((int **)&dps.bx_ptr, ps.bx_ptr));
+        PFCHK(upload((int **)&dps.bx_i, ps.bx_i));
+        PFCHK(upload((int **)&dps.bx_p, ps.bx_p));
+        PFCHK(allow((const void *)k_fwd_parts_lds<double>,
+                    std::max<long long>(part_sweep_lds<double>(ps.max_nodes, ps.max_fi, false), 16), psw_lds[0][0]));
+        PFCHK(allow((const void *)k_bwd_parts_lds<double>,
+                    std::max<long long>(part_sweep_lds<double>(ps.max_nodes, ps.max_bi, true), 16), psw_lds[0][1]));
+        PFCHK(allow((const void *)k_fwd_parts_lds<float>,
+                    std::max<long long>(part_sweep_lds<float>(ps.max_nodes, ps.max_fi, false), 16), psw_lds[1][0]));
+        PFCHK(allow((const void *)k_bwd_parts_lds<float>,
+                    std::max<long long>(part_sweep_lds<float>(ps.max_nodes, ps.max_bi, true), 16), psw_lds[1][1]));
         const FlatTail fp = flat_parts(sym);
         PFCHK(upload_flat(fp, dfp));
         PFCHK(allow((const void *)k_factor_parts_flat<double>,
@@ -517,7 +541,10 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
 
 template <typename T>
 pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
-    if (sym.n_parts > 0) {
+    const int *pl = psw_lds[sizeof(T) == 4 ? 1 : 0];
+    if (sym.n_parts > 0 && pl[0]) {
+        k_fwd_parts_lds<T><<<sym.n_parts, 256, pl[0], stream>>>(dy, dps, blk, v, ctl, gate);
+    } else if (sym.n_parts > 0) {
         k_fwd_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, v, ctl, gate);
     } else {
         for (int l = 0; l < sym.tail_start; l++) {
@@ -540,7 +567,9 @@ pf_status pf_solver::enqueue_solve(const T *blk, double *v, int gate) {
         if (sym.tail_start < sym.nlev_s)
             k_bwd_tail<T><<<1, 1024, 0, stream>>>(dy, blk, v, ctl, sym.tail_start, sym.nlev_s, gate);
     }
-    if (sym.n_parts > 0) {
+    if (sym.n_parts > 0 && pl[1]) {
+        k_bwd_parts_lds<T><<<sym.n_parts, 256, pl[1], stream>>>(dy, dps, blk, v, ctl, gate);
+    } else if (sym.n_parts > 0) {
         k_bwd_parts<T><<<sym.n_parts, 1024, 0, stream>>>(dy, blk, v, ctl, gate);
     } else {
         for (int l = sym.tail_start - 1; l >= 0; l--) {

@@ -513,6 +513,50 @@ FlatTail flat_parts(const Symbolic &s) {
     return f;
 }
 
+PartSweeps part_sweeps(const Symbolic &s) {
+    PartSweeps ps;
+    const int T = s.tail_start;
+    std::vector<int> loc(s.n, -1), grp(s.n, -1);
+    for (int g = 0; g < s.n_parts; g++) {
+        const int u0 = s.part_ptr[(size_t)g * T], u1 = s.part_ptr[(size_t)(g + 1) * T];
+        for (int u = u0; u < u1; u++) {
+            loc[s.part_nodes[u]] = u - u0;
+            grp[s.part_nodes[u]] = g;
+        }
+        ps.max_nodes = std::max(ps.max_nodes, u1 - u0);
+    }
+    ps.fi_ptr.assign(1, 0);
+    ps.bi_ptr.assign(1, 0);
+    ps.bx_ptr.assign(1, 0);
+    for (int g = 0; g < s.n_parts; g++) {
+        const int u0 = s.part_ptr[(size_t)g * T], u1 = s.part_ptr[(size_t)(g + 1) * T];
+        const int f0 = (int)ps.fi_k.size(), b0 = (int)ps.bi_i.size();
+        for (int u = u0; u < u1; u++) {
+            const int j = s.part_nodes[u];
+            for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) {   // sources: the group's own nodes
+                ps.fi_k.push_back(loc[s.rs_k[q]]);
+                ps.fi_q.push_back(q);
+            }
+            for (int p = s.colptr[j]; p < s.colptr[j + 1]; p++) {
+                const int i = s.rowind[p];
+                if (grp[i] == g) {
+                    ps.bi_i.push_back(loc[i]);
+                    ps.bi_p.push_back(p);
+                } else {
+                    ps.bx_i.push_back(i);
+                    ps.bx_p.push_back(p);
+                }
+            }
+            ps.fi_ptr.push_back((int)ps.fi_k.size());
+            ps.bi_ptr.push_back((int)ps.bi_i.size());
+            ps.bx_ptr.push_back((int)ps.bx_i.size());
+        }
+        ps.max_fi = std::max(ps.max_fi, (int)ps.fi_k.size() - f0);
+        ps.max_bi = std::max(ps.max_bi, (int)ps.bi_i.size() - b0);
+    }
+    return ps;
+}
+
 // ------------------------------------------------------------- host numeric
 
 namespace {
@@ -702,18 +746,33 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
     return true;
 }
 
-void host_split_solve(const Symbolic &s, const TailSweeps &ts, const double *blk, double *x) {
-    const int n = s.n, L0 = n, U0 = n + s.nnzL, t0 = ts.t0, t1 = ts.t1;
+void host_split_solve(const Symbolic &s, const TailSweeps &ts, const PartSweeps &ps, const double *blk,
+                      double *x) {
+    const int n = s.n, L0 = n, U0 = n + s.nnzL, t0 = ts.t0, t1 = ts.t1, T = s.tail_start;
     auto Lr = [&](int q) { return blk + 4 * (size_t)(L0 + s.rs_p[q]); };   // L by row position q
     auto sub = [&](double *y, const double *B, int k) {
         y[0] -= B[0] * x[2 * k] + B[1] * x[2 * k + 1];
         y[1] -= B[2] * x[2 * k] + B[3] * x[2 * k + 1];
     };
-    for (int j = 0; j < t0; j++) {   // below the top: every row entry
-        double y[2] = {x[2 * j], x[2 * j + 1]};
-        for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) sub(y, Lr(q), s.rs_k[q]);
-        x[2 * j] = y[0];
-        x[2 * j + 1] = y[1];
+    const bool groups = s.n_parts > 0 && !ps.fi_ptr.empty();
+    if (groups) {   // below the top: each subtree group level by level, entries by local index
+        for (int g = 0; g < s.n_parts; g++) {
+            const int u0 = s.part_ptr[(size_t)g * T];
+            for (int u = u0; u < s.part_ptr[(size_t)(g + 1) * T]; u++) {   // level-major within the group
+                const int j = s.part_nodes[u];
+                double y[2] = {x[2 * j], x[2 * j + 1]};
+                for (int e = ps.fi_ptr[u]; e < ps.fi_ptr[u + 1]; e++) sub(y, Lr(ps.fi_q[e]), s.part_nodes[u0 + ps.fi_k[e]]);
+                x[2 * j] = y[0];
+                x[2 * j + 1] = y[1];
+            }
+        }
+    } else {
+        for (int j = 0; j < t0; j++) {   // below the top: every row entry
+            double y[2] = {x[2 * j], x[2 * j + 1]};
+            for (int q = s.rs_ptr[j]; q < s.rs_ptr[j + 1]; q++) sub(y, Lr(q), s.rs_k[q]);
+            x[2 * j] = y[0];
+            x[2 * j + 1] = y[1];
+        }
     }
     for (int a = 0; a < t1 - t0; a++) {   // the top: entries below it first
         double y[2] = {x[2 * (t0 + a)], x[2 * (t0 + a) + 1]};
@@ -767,6 +826,29 @@ void host_split_solve(const Symbolic &s, const TailSweeps &ts, const double *blk
         const double *Di = blk + 4 * (size_t)j;
         x[2 * j] = Di[0] * v[0] + Di[1] * v[1];
         x[2 * j + 1] = Di[2] * v[0] + Di[3] * v[1];
+    }
+    if (groups) {   // each group: the entries above it first, then its own in reverse
+        for (int g = 0; g < s.n_parts; g++) {
+            const int u0 = s.part_ptr[(size_t)g * T], u1 = s.part_ptr[(size_t)(g + 1) * T];
+            std::vector<double> yg(2 * (size_t)(u1 - u0));
+            for (int u = u0; u < u1; u++) {
+                const int j = s.part_nodes[u];
+                double v[2] = {x[2 * j], x[2 * j + 1]};
+                for (int e = ps.bx_ptr[u]; e < ps.bx_ptr[u + 1]; e++) sub(v, blk + 4 * (size_t)(U0 + ps.bx_p[e]), ps.bx_i[e]);
+                yg[2 * (u - u0)] = v[0];
+                yg[2 * (u - u0) + 1] = v[1];
+            }
+            for (int u = u1 - 1; u >= u0; u--) {
+                const int j = s.part_nodes[u];
+                double v[2] = {yg[2 * (u - u0)], yg[2 * (u - u0) + 1]};
+                for (int e = ps.bi_ptr[u]; e < ps.bi_ptr[u + 1]; e++)
+                    sub(v, blk + 4 * (size_t)(U0 + ps.bi_p[e]), s.part_nodes[u0 + ps.bi_i[e]]);
+                const double *Di = blk + 4 * (size_t)j;
+                x[2 * j] = Di[0] * v[0] + Di[1] * v[1];
+                x[2 * j + 1] = Di[2] * v[0] + Di[3] * v[1];
+            }
+        }
+        return;
     }
     for (int j = t0 - 1; j >= 0; j--) {
         double v[2] = {x[2 * j], x[2 * j + 1]};

@@ -933,7 +933,7 @@ __device__ __forceinline__ void wg_nr(const BatchTopo &tp, const BatchJob &J, Wo
             for (int l = 0; l < Y.nlev_s;) {
                 if (s_ck && tp.F.fl_ptr && l == tp.F.lv0 && ((size_t)w.x & 15) == 0) {   // flat pair lists
                     T *pa = reinterpret_cast<T *>(w.x);   // x and r: free until the solve
-                    flat_factor_levels<T, 1, 4>(Y, tp.F, w.blk, 0, Y.nlev_s - tp.F.lv0, &s_sing, pa,
+                    flat_factor_levels<T, 1>(Y, tp.F, w.blk, 0, Y.nlev_s - tp.F.lv0, &s_sing, pa,
                                                 pa + 4 * (size_t)tp.F.max_pairs);
                     break;
                 }
@@ -1416,6 +1416,7 @@ pf_status pf_batch::add_net(const pf_network &src) {
             F.lv0 = ft.lv0;
             F.max_pairs = ft.max_pairs;
             F.max_col = ft.max_col;
+            F.de_sentinel = ft.de_ptr.back();
             BCHK(upload((int **)&F.fl_ptr, ft.fl_ptr));
             BCHK(upload((int **)&F.fl_split, ft.fl_split));
             BCHK(upload((int **)&F.fl_a, ft.fl_a));
@@ -1429,6 +1430,8 @@ pf_status pf_batch::add_net(const pf_network &src) {
             BCHK(upload((int **)&F.de_hi, ft.de_hi));
             BCHK(upload((int **)&F.de_dlo, ft.de_dlo));
             BCHK(upload((int **)&F.de_dhi, ft.de_dhi));
+            BCHK(upload((int **)&F.de_lr, ft.de_lr));
+            BCHK(upload((int **)&F.st_g, ft.st_g));
         }
     }
     BCHK(upload((double **)&b->tp.V0flat, net.V));

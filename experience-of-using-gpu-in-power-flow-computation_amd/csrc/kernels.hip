@@ -379,7 +379,7 @@ __global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, 
     if (gate && !ctl->step) return;
     extern __shared__ __align__(16) char smem[];
     T *s_pa = reinterpret_cast<T *>(smem);
-    flat_factor_levels<T, sizeof(T) == 4 ? 3 : 2, sizeof(T) == 4 ? 8 : 4>(Y, F, blk, 0, nsteps, &ctl->singular, s_pa,
+    flat_factor_levels<T, sizeof(T) == 4 ? 3 : 1>(Y, F, blk, 0, nsteps, &ctl->singular, s_pa,
                                                                             s_pa + 4 * (size_t)F.max_pairs);
 }
 
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(256) k_factor_parts_flat(DevSym Y, DevFlat F, 
     extern __shared__ __align__(16) char smem[];
     T *s_pa = reinterpret_cast<T *>(smem);
     const int t0 = (int)blockIdx.x * Y.part_levels;
-    flat_factor_levels<T, 2, 4>(Y, F, blk, t0, t0 + Y.part_levels, &ctl->singular, s_pa,
+    flat_factor_levels<T, 2>(Y, F, blk, t0, t0 + Y.part_levels, &ctl->singular, s_pa,
                                 s_pa + 4 * (size_t)F.max_pairs);
 }
 

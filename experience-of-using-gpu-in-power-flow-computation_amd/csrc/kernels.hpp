@@ -118,9 +118,9 @@ struct DevTail {
 // each its destinations [de_ptr[t], de_ptr[t+1]) and pairs
 // [fl_ptr[t], fl_ptr[t+1]); lv0: the first level of a level-by-level program.
 struct DevFlat {
-    int lv0, max_pairs, max_col;
+    int lv0, max_pairs, max_col, de_sentinel;
     const int *fl_ptr, *fl_split, *fl_a, *fl_b, *de_ptr;
-    const int *de_node, *de_p, *de_q0, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi;
+    const int *de_node, *de_p, *de_q0, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi, *de_lr, *st_g;
 };
 
 // The subtree groups' sweeps with their unknowns in LDS (PartSweeps on the

@@ -337,116 +337,222 @@ __device__ __forceinline__ B4<T> lds_sum_in_order(const T *v, int lo, int hi) {
     return a;
 }
 
+// v + v of the lane that the DPP control selects within the row of 16 lanes
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return v + __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum of an aligned group of g = 4 or 16 lanes' values p_0 .. p_{g-1} as the
+// balanced tree in lane order: (p0 + p1) + (p2 + p3) in each quad, then (g =
+// 16) the quads in pairs (row_half_mirror pairs lane i with 7 - i) and the
+// halves (row_mirror: i with 15 - i).  Every step adds two values that are
+// the same bits in both lanes of a pair, so every lane ends with the same
+// sum.  g is uniform over the wave.
+template <typename T>
+__device__ __forceinline__ B4<T> group_tree_sum(B4<T> v, int g) {
+    v = B4<T>{dpp_add<0xB1>(v.a), dpp_add<0xB1>(v.b), dpp_add<0xB1>(v.c), dpp_add<0xB1>(v.d)};   // lane ^ 1
+    v = B4<T>{dpp_add<0x4E>(v.a), dpp_add<0x4E>(v.b), dpp_add<0x4E>(v.c), dpp_add<0x4E>(v.d)};   // lane ^ 2
+    if (g == 16) {
+        v = B4<T>{dpp_add<0x141>(v.a), dpp_add<0x141>(v.b), dpp_add<0x141>(v.c), dpp_add<0x141>(v.d)};
+        v = B4<T>{dpp_add<0x140>(v.a), dpp_add<0x140>(v.b), dpp_add<0x140>(v.c), dpp_add<0x140>(v.d)};
+    }
+    return v;
+}
+
+// a load from an array that stays constant while the kernel runs: through
+// the constant address space, so that a uniform index is a scalar load
+__device__ __forceinline__ int ld_const(const int *p, int i) {
+    return ((const __attribute__((address_space(4))) int *)p)[i];
+}
+
+template <typename T>
+__device__ __forceinline__ B4<T> add4(const B4<T> &a, const B4<T> &x) {
+    return B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
+}
+template <typename T>
+__device__ __forceinline__ B4<T> sub4(const B4<T> &a, const B4<T> &x) {
+    return B4<T>{a.a - x.a, a.b - x.b, a.c - x.c, a.d - x.d};
+}
+
 // Level steps [t0, t1) of a flat pair program (DevFlat) in one workgroup,
 // two barriers per step:
 //   A. one update pair per lane (strided), its block product(s) into LDS
 //      (s_pa; s_pu for the column destinations' second product);
-//   B. one thread per destination adds its products in pair order; a pivot
-//      thread stores the inverse of the updated pivot, a column thread
+//   B. st_g[t] lanes per destination add its products (lane i: products
+//      i, i + G, ... in order, then the lanes' sums as a balanced tree); a
+//      pivot stores the inverse of its updated pivot, a column destination
 //      forms the same updated pivot from the pivot's products, inverts it
 //      and stores its L block scaled by it (both L copies) and its U block.
 //      The pivot slots are left as they were (nothing after the
 //      factorisation reads them; the inverse pivots carry the result).
-// Only A waits on global loads, one round of them: a lane's first R pairs
-// of a step have their indices in registers from the step before, and all
-// their blocks are requested before the first product (a pair past the
-// step's end reads block 0 and is not stored).  B sums W products per round
-// of LDS loads.  s_pa and s_pu hold F.max_pairs and F.max_col blocks.
-template <typename T, int R, int W>
-__device__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, int t0, int t1, int *bad, T *s_pa,
+// Only A waits on global loads, one round of them.  Everything whose
+// address does not depend on this step's results is requested a step
+// ahead: the next step's pair indices and its first destination's indices
+// (this step's are in registers), the metadata of the step after that.  A
+// step's own loads (its pairs' blocks and its destination's blocks, which
+// nothing changes before this step) are then issued together and only they
+// are waited for.  Every such load is unconditional: a lane without a pair
+// reads pair 0's slot, a lane without a destination reads the program's
+// sentinel destination (a pivot of node 0 without pairs), and a step past
+// t1 is empty.  s_pa and s_pu hold F.max_pairs and F.max_col blocks.
+template <typename T, int R>
+__device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, int t0, int t1, int *bad, T *s_pa,
                                    T *s_pu) {
+    if (t0 >= t1) return;
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    struct Step {
+        int base, np, ns, e0, ne, g;
+    };
+    // the program's arrays are constant while it runs: their uniform loads
+    // go through the scalar cache
+    auto step_at = [&](int t) __attribute__((always_inline)) {
+        const bool in = t < t1;
+        const int tc = in ? t : t1 - 1;
+        Step S;
+        S.base = ld_const(F.fl_ptr, tc);
+        S.np = in ? ld_const(F.fl_ptr, tc + 1) - S.base : 0;
+        S.ns = ld_const(F.fl_split, tc);
+        S.e0 = ld_const(F.de_ptr, tc);
+        S.ne = in ? ld_const(F.de_ptr, tc + 1) - S.e0 : 0;
+        S.g = ld_const(F.st_g, tc);
+        return S;
+    };
+    auto lane_group = [](int g, int x) { return g == 1 ? x : g == 4 ? x >> 2 : x >> 4; };
     int pa[R], pb[R];
-    auto pairs_at = [&](int t, int *qa, int *qb) __attribute__((always_inline)) {
-        const int b = F.fl_ptr[t], np = F.fl_ptr[t + 1] - b;
+    auto pairs_of = [&](const Step &S) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int e = tid + r * bd;
-            qa[r] = e < np ? F.fl_a[b + e] : 0;
-            qb[r] = e < np ? F.fl_b[b + e] : 0;
+            const bool in = e < S.np;
+            const int q = in ? S.base + e : S.base;   // the sentinel pair when the step is empty
+            const int x = F.fl_a[q], y = F.fl_b[q];
+            pa[r] = in ? x : 0;
+            pb[r] = in ? y : 0;
         }
     };
-    struct Dest {
-        int j, p, lo, hi, dlo, dhi, lrp;
-        B4<T> v0, v1, vd;   // current blocks: own (L or pivot), U, and the node's pivot
+    struct Idx {
+        int j, p, lo, hi, dlo, dhi, lr;
     };
-    auto dest = [&](int E) __attribute__((always_inline)) {
-        Dest D;
-        D.j = F.de_node[E];
-        D.p = F.de_p[E];
-        D.lo = F.de_lo[E];
-        D.hi = F.de_hi[E];
-        D.dlo = F.de_dlo[E];
-        D.dhi = F.de_dhi[E];
-        D.vd = load_b4(blk, D.j);
-        D.v0 = D.vd;
-        D.v1 = B4<T>{0, 0, 0, 0};
-        D.lrp = 0;
-        if (D.p >= 0) {
-            D.v0 = load_b4(blk, L0 + D.p);
-            D.v1 = load_b4(blk, U0 + D.p);
-            D.lrp = Y.lr_pos[D.p];
+    struct Val {
+        B4<T> v0, v1, vd;   // own block (L, or the pivot), U block (a pivot: its pivot again), node's pivot
+    };
+    auto idx_of = [&](int E) __attribute__((always_inline)) {
+        return Idx{F.de_node[E], F.de_p[E], F.de_lo[E], F.de_hi[E], F.de_dlo[E], F.de_dhi[E], F.de_lr[E]};
+    };
+    auto first_dest = [&](const Step &S) __attribute__((always_inline)) {
+        const int me = lane_group(S.g, tid);
+        return me < S.ne ? S.e0 + me : F.de_sentinel;
+    };
+    auto val_of = [&](const Idx &D) __attribute__((always_inline)) {
+        const bool col = D.p >= 0;
+        return Val{load_b4(blk, col ? L0 + D.p : D.j), load_b4(blk, col ? U0 + D.p : D.j), load_b4(blk, D.j)};
+    };
+    // destination D of a step with g lanes per destination: lane i adds
+    // the products i, i + g, ... of its three sums (a: own, ad: the node's
+    // pivot's, u: the U block's) in order, two at a time with their loads
+    // in flight together; the group then adds its lanes' sums as a tree and
+    // its first lane stores the pivot inverse or the scaled L and the U block
+    auto dest_b = [&](int g, int ns, const Idx &D, const Val &V) __attribute__((always_inline)) {
+        const int lane = tid & (g - 1);
+        const bool col = D.p >= 0;
+        const int na = D.hi - D.lo, nd = col ? D.dhi - D.dlo : 0;
+        const int nr = na > nd ? na : nd;
+        B4<T> a{0, 0, 0, 0}, ad{0, 0, 0, 0}, u{0, 0, 0, 0};
+        for (int q = lane; q < nr; q += 2 * g) {
+            const int q2 = q + g;
+            const bool a1 = q < na, a2 = q2 < na, d1 = q < nd, d2 = q2 < nd;
+            B4<T> xa1{0, 0, 0, 0}, xa2{0, 0, 0, 0}, xu1{0, 0, 0, 0}, xu2{0, 0, 0, 0}, xd1{0, 0, 0, 0},
+                xd2{0, 0, 0, 0};
+            if (a1) xa1 = load_b4(s_pa, D.lo + q);
+            if (a2) xa2 = load_b4(s_pa, D.lo + q2);
+            if (col && a1) xu1 = load_b4(s_pu, D.lo - ns + q);
+            if (col && a2) xu2 = load_b4(s_pu, D.lo - ns + q2);
+            if (d1) xd1 = load_b4(s_pa, D.dlo + q);
+            if (d2) xd2 = load_b4(s_pa, D.dlo + q2);
+            if (a1) {
+                a = add4(a, xa1);
+                u = add4(u, xu1);
+            }
+            if (a2) {
+                a = add4(a, xa2);
+                u = add4(u, xu2);
+            }
+            if (d1) ad = add4(ad, xd1);
+            if (d2) ad = add4(ad, xd2);
         }
-        return D;
+        if (g > 1) {
+            a = group_tree_sum(a, g);
+            if (col) {
+                ad = group_tree_sum(ad, g);
+                u = group_tree_sum(u, g);
+            }
+        }
+        if (lane != 0) return;
+        const B4<T> P = col ? sub4(V.vd, ad) : sub4(V.v0, a);
+        T det = P.a * P.d - P.b * P.c;
+        const bool sing = !(det != (T)0) || !isfinite((double)det);
+        if (sing) det = (T)1;
+        const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
+        if (!col) {   // the pivot slot keeps its value: B of a later round reads it
+            if (sing) atomicOr(bad, 1);
+            store_b4(blk, DI + D.j, Pi);
+        } else {
+            const B4<T> lb = mul(sub4(V.v0, a), Pi);
+            store_b4(blk, L0 + D.p, lb);
+            store_b4(blk, LR0 + D.lr, lb);
+            store_b4(blk, U0 + D.p, sub4(V.v1, u));
+        }
     };
-    if (t0 < t1) pairs_at(t0, pa, pb);
+    Step S = step_at(t0), S1 = step_at(t0 + 1);
+    pairs_of(S);
+    Idx nx = idx_of(first_dest(S));   // this thread's first destination of the coming step
     for (int t = t0; t < t1; t++) {
-        const int base = F.fl_ptr[t], np = F.fl_ptr[t + 1] - base, ns = F.fl_split[t];
-        const int e0 = F.de_ptr[t], ne = F.de_ptr[t + 1] - e0;
-        Dest mine{};   // this thread's first destination, read during A
-        if (tid < ne) mine = dest(e0 + tid);
-        {
-            B4<T> la[R], ub[R], lb[R], ua[R];
+        const Idx mi = nx;
+        const Val mv = val_of(mi);
+        B4<T> la[R], ub[R], lb[R], ua[R];
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                la[r] = load_b4(blk, L0 + pa[r]);
-                ub[r] = load_b4(blk, U0 + pb[r]);
-                lb[r] = load_b4(blk, L0 + pb[r]);
-                ua[r] = load_b4(blk, U0 + pa[r]);
-            }
-            if (t + 1 < t1) pairs_at(t + 1, pa, pb);
+        for (int r = 0; r < R; r++) {
+            la[r] = load_b4(blk, L0 + pa[r]);
+            ub[r] = load_b4(blk, U0 + pb[r]);
+            lb[r] = load_b4(blk, L0 + pb[r]);
+            ua[r] = load_b4(blk, U0 + pa[r]);
+        }
+        pairs_of(S1);
+        nx = idx_of(first_dest(S1));
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                const int e = tid + r * bd;
-                if (e < np) {
-                    store_b4(s_pa, e, mul(la[r], ub[r]));
-                    if (e >= ns) store_b4(s_pu, e - ns, mul(lb[r], ua[r]));
-                }
+        for (int r = 0; r < R; r++) {
+            const int e = tid + r * bd;
+            if (e < S.np) {
+                store_b4(s_pa, e, mul(la[r], ub[r]));
+                if (e >= S.ns) store_b4(s_pu, e - S.ns, mul(lb[r], ua[r]));
             }
         }
-        for (int e = tid + R * bd; e < np; e += bd) {
-            const int qa = F.fl_a[base + e], qb = F.fl_b[base + e];
+        for (int e = tid + R * bd; e < S.np; e += bd) {
+            const int qa = F.fl_a[S.base + e], qb = F.fl_b[S.base + e];
             store_b4(s_pa, e, mul(load_b4(blk, L0 + qa), load_b4(blk, U0 + qb)));
-            if (e >= ns) store_b4(s_pu, e - ns, mul(load_b4(blk, L0 + qb), load_b4(blk, U0 + qa)));
+            if (e >= S.ns) store_b4(s_pu, e - S.ns, mul(load_b4(blk, L0 + qb), load_b4(blk, U0 + qa)));
         }
         __syncthreads();
-        for (int e = tid; e < ne; e += bd) {
-            const Dest D = (e == tid) ? mine : dest(e0 + e);
-            const B4<T> a = lds_sum_in_order<T, W>(s_pa, D.lo, D.hi);
-            B4<T> P;
-            if (D.p < 0) {
-                P = B4<T>{D.v0.a - a.a, D.v0.b - a.b, D.v0.c - a.c, D.v0.d - a.d};
-            } else {
-                const B4<T> ad = lds_sum_in_order<T, W>(s_pa, D.dlo, D.dhi);
-                P = B4<T>{D.vd.a - ad.a, D.vd.b - ad.b, D.vd.c - ad.c, D.vd.d - ad.d};
-            }
-            T det = P.a * P.d - P.b * P.c;
-            const bool sing = !(det != (T)0) || !isfinite((double)det);
-            if (sing) det = (T)1;
-            const B4<T> Pi{P.d / det, -P.b / det, -P.c / det, P.a / det};
-            if (D.p < 0) {   // the pivot slot keeps its value: B of a later round reads it
-                if (sing) atomicOr(bad, 1);
-                store_b4(blk, DI + D.j, Pi);
-            } else {
-                const B4<T> u = lds_sum_in_order<T, W>(s_pu, D.lo - ns, D.hi - ns);
-                const B4<T> lb = mul(B4<T>{D.v0.a - a.a, D.v0.b - a.b, D.v0.c - a.c, D.v0.d - a.d}, Pi);
-                store_b4(blk, L0 + D.p, lb);
-                store_b4(blk, LR0 + D.lrp, lb);
-                store_b4(blk, U0 + D.p, B4<T>{D.v1.a - u.a, D.v1.b - u.b, D.v1.c - u.c, D.v1.d - u.d});
+        {   // B: the thread's first destination from registers, further ones (wide steps) loaded here
+            const int grp = lane_group(S.g, tid), ng = lane_group(S.g, bd);
+            if (grp < S.ne) dest_b(S.g, S.ns, mi, mv);
+            for (int e = grp + ng; e < S.ne; e += ng) {
+                const Idx D = idx_of(S.e0 + e);
+                dest_b(S.g, S.ns, D, val_of(D));
             }
         }
         __syncthreads();
+        S = S1;
+        S1 = step_at(t + 2);
     }
 }
 

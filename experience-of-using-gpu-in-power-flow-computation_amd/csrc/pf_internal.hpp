@@ -152,8 +152,15 @@ TailSweeps tail_sweeps(const Symbolic &s);
 // de_node[E], L position de_p[E] (-1: the pivot), its pairs at
 // [de_lo[E], de_hi[E]) of its step's list and those of its node's pivot at
 // [de_dlo[E], de_dhi[E]); its pairs from sources below the program, applied
-// before it, are fc_a/fc_b[de_q0[E], de_ext[E]).  max_pairs / max_col bound
-// one step.
+// before it, are fc_a/fc_b[de_q0[E], de_ext[E]); de_lr[E] = lr_pos of its
+// L position (0 for a pivot).  A destination's products are summed by
+// st_g[t] lanes (1, 4 or 16, a function of the step alone: see
+// flat_group_width): lane i adds products lo + i, lo + i + G, ... in order
+// and the lanes' sums are added as a balanced tree in lane order
+// (flat_group_sum restates it).  max_pairs / max_col bound one step.  The
+// pair and destination lists end with one sentinel entry past the program
+// (pair 0; a pivot of node 0 without pairs), which lanes without a pair or
+// a destination read when they prefetch; de_ptr.back() is its index.
 //   flat_tail: levels [lv0, nlev_s), step t = level lv0 + t, destinations
 //   in ld order; with split_below (single-network solver, lv0 = tail_start)
 //   the pairs from sources below lv0 (k < lvl_ptr[lv0]) are left to a
@@ -164,11 +171,16 @@ TailSweeps tail_sweeps(const Symbolic &s);
 struct FlatTail {
     int lv0 = 0;
     std::vector<int> fl_ptr, fl_split, fl_a, fl_b, de_ptr;
-    std::vector<int> de_node, de_p, de_q0, de_ext, de_lo, de_hi, de_dlo, de_dhi;
+    std::vector<int> de_node, de_p, de_q0, de_ext, de_lo, de_hi, de_dlo, de_dhi, de_lr, st_g;
     int max_pairs = 0, max_col = 0;
 };
 FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below);
 FlatTail flat_parts(const Symbolic &s);
+// lanes per destination of a step with nd destinations of at most maxpd
+// products each
+int flat_group_width(int nd, int maxpd);
+// the sum of v[lo, hi) (blocks of 4 doubles) in the order G lanes take it
+void flat_group_sum(const double *v, int lo, int hi, int G, double *out);
 
 // The subtree groups' sweeps with each group's unknowns held by its
 // workgroup.  Per node u of part_nodes (group g owns u in [part_ptr[g T],

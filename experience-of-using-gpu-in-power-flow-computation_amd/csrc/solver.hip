@@ -438,6 +438,7 @@ pf_status pf_solver::upload_flat(const FlatTail &ft, DevFlat &F) {
     F.lv0 = ft.lv0;
     F.max_pairs = ft.max_pairs;
     F.max_col = ft.max_col;
+    F.de_sentinel = ft.de_ptr.back();
     PFCHK(upload((int **)&F.fl_ptr, ft.fl_ptr));
     PFCHK(upload((int **)&F.fl_split, ft.fl_split));
     PFCHK(upload((int **)&F.fl_a, ft.fl_a));
@@ -451,6 +452,8 @@ pf_status pf_solver::upload_flat(const FlatTail &ft, DevFlat &F) {
     PFCHK(upload((int **)&F.de_hi, ft.de_hi));
     PFCHK(upload((int **)&F.de_dlo, ft.de_dlo));
     PFCHK(upload((int **)&F.de_dhi, ft.de_dhi));
+    PFCHK(upload((int **)&F.de_lr, ft.de_lr));
+    PFCHK(upload((int **)&F.st_g, ft.st_g));
     return PF_OK;
 }
 
@@ -467,7 +470,7 @@ pf_status pf_solver::setup_flat_tail() {
         return PF_OK;
     };
     const FlatTail ft = flat_tail(sym, sym.tail_start, true);
-    flat_ne = (int)ft.de_node.size();
+    flat_ne = ft.de_ptr.back();
     PFCHK(upload_flat(ft, dfl));
     PFCHK(allow((const void *)k_factor_tail_flat<double>,
                 std::max<long long>(flat_tail_lds<double>(ft.max_pairs, ft.max_col), 16), flat_lds[0]));

@@ -431,6 +431,27 @@ TailSweeps tail_sweeps(const Symbolic &s) {
     return t;
 }
 
+// Narrow steps with long product lists take 16 or 4 lanes per destination:
+// a sum of up to ~40 products then costs a few LDS loads per lane and a
+// short tree instead of a chain of dependent additions in one lane.  The
+// width depends on the program only, so every launch shape sums alike.
+int flat_group_width(int nd, int maxpd) {
+    if (nd <= 64 && maxpd >= 16) return 16;
+    if (nd <= 256 && maxpd >= 6) return 4;
+    return 1;
+}
+
+void flat_group_sum(const double *v, int lo, int hi, int G, double *out) {
+    double part[16][4] = {};
+    for (int l = 0; l < G; l++)
+        for (int q = lo + l; q < hi; q += G)
+            for (int c = 0; c < 4; c++) part[l][c] += v[4 * (size_t)q + c];
+    for (int w = 1; w < G; w *= 2)   // (p0 + p1) + (p2 + p3), then pairs of those, ...
+        for (int l = 0; l < G; l += 2 * w)
+            for (int c = 0; c < 4; c++) part[l][c] += part[l + w][c];
+    for (int c = 0; c < 4; c++) out[c] = part[0][c];
+}
+
 namespace {
 
 // one level step of a flat program: the destinations of the given nodes,
@@ -449,6 +470,7 @@ void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0,
                 while (q < s.fc_ptr[d + 1] && pcol[s.fc_b[q]] < t0) q++;   // pairs ascend in k
                 f.de_node.push_back(j);
                 f.de_p.push_back(t == 0 ? -1 : s.colptr[j] + t - 1);
+                f.de_lr.push_back(t == 0 ? 0 : s.lr_pos[s.colptr[j] + t - 1]);
                 f.de_q0.push_back(s.fc_ptr[d]);
                 f.de_ext.push_back(q);
                 f.de_lo.push_back((int)f.fl_a.size() - base);
@@ -467,10 +489,28 @@ void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0,
         }
     }
     const int np = (int)f.fl_a.size() - base;
+    const int E0 = f.de_ptr.back(), E1 = (int)f.de_node.size();
+    int maxpd = 0;
+    for (int E = E0; E < E1; E++) maxpd = std::max(maxpd, f.de_hi[E] - f.de_lo[E]);
+    f.st_g.push_back(flat_group_width(E1 - E0, maxpd));
     f.max_pairs = std::max(f.max_pairs, np);
     f.max_col = std::max(f.max_col, np - f.fl_split.back());
     f.fl_ptr.push_back((int)f.fl_a.size());
     f.de_ptr.push_back((int)f.de_node.size());
+}
+
+void flat_seal(FlatTail &f) {
+    f.fl_a.push_back(0);
+    f.fl_b.push_back(0);
+    f.de_node.push_back(0);
+    f.de_p.push_back(-1);
+    f.de_q0.push_back(0);
+    f.de_ext.push_back(0);
+    f.de_lo.push_back(0);
+    f.de_hi.push_back(0);
+    f.de_dlo.push_back(0);
+    f.de_dhi.push_back(0);
+    f.de_lr.push_back(0);
 }
 
 std::vector<int> position_columns(const Symbolic &s) {
@@ -495,6 +535,7 @@ FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below) {
         for (int j = s.lvl_ptr[l]; j < s.lvl_ptr[l + 1]; j++) nodes.push_back(j);
         flat_step(s, f, nodes.data(), (int)nodes.size(), t0, pcol, dlo, dhi);
     }
+    flat_seal(f);
     return f;
 }
 
@@ -510,6 +551,7 @@ FlatTail flat_parts(const Symbolic &s) {
             const int a = s.part_ptr[(size_t)g * T + l], b = s.part_ptr[(size_t)g * T + l + 1];
             flat_step(s, f, s.part_nodes.data() + a, b - a, 0, pcol, dlo, dhi);
         }
+    flat_seal(f);
     return f;
 }
 
@@ -680,15 +722,13 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
     std::vector<double> inv(4 * (size_t)n, 0.0);   // inverse pivots; the pivot slots keep A_jj
     auto L = [&](int p) { return blk + 4 * (size_t)(L0 + p); };
     auto U = [&](int p) { return blk + 4 * (size_t)(U0 + p); };
-    auto sum = [](const std::vector<double> &v, int lo, int hi, double *a) {
-        a[0] = a[1] = a[2] = a[3] = 0.0;
-        for (int q = lo; q < hi; q++)
-            for (int c = 0; c < 4; c++) a[c] += v[4 * (size_t)q + c];
-    };
     auto run = [&](const FlatTail &f) -> bool {
         const int nsteps = (int)f.fl_ptr.size() - 1;
         for (int t = 0; t < nsteps; t++) {
-            const int base = f.fl_ptr[t], np = f.fl_ptr[t + 1] - base, ns = f.fl_split[t];
+            const int base = f.fl_ptr[t], np = f.fl_ptr[t + 1] - base, ns = f.fl_split[t], G = f.st_g[t];
+            auto sum = [G](const std::vector<double> &v, int lo, int hi, double *a) {
+                flat_group_sum(v.data(), lo, hi, G, a);
+            };
             std::vector<double> pa(4 * (size_t)np), pu(4 * (size_t)std::max(np - ns, 0));
             for (int e = 0; e < np; e++) {   // every product from earlier steps' blocks
                 mul2(&pa[4 * (size_t)e], L(f.fl_a[base + e]), U(f.fl_b[base + e]));
@@ -724,7 +764,7 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
     };
     if (!parts.fl_ptr.empty() && !run(parts)) return false;
     // the narrow top's pairs from below it, applied first
-    for (size_t E = 0; E < top.de_node.size(); E++) {
+    for (int E = 0; E < (top.de_ptr.empty() ? 0 : top.de_ptr.back()); E++) {
         const int j = top.de_node[E], p = top.de_p[E];
         double a[4] = {0, 0, 0, 0}, u[4] = {0, 0, 0, 0}, t4[4];
         for (int q = top.de_q0[E]; q < top.de_ext[E]; q++) {

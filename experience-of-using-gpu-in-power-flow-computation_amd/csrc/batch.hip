@@ -338,11 +338,8 @@ __device__ __forceinline__ void wg_solve(const DevSym &Y, const T *blk, double *
                 a0 += (double)L.a * x[2 * k] + (double)L.b * x[2 * k + 1];
                 a1 += (double)L.c * x[2 * k] + (double)L.d * x[2 * k + 1];
             }
-#pragma unroll
-            for (int off = GL / 2; off > 0; off >>= 1) {
-                a0 += __shfl_xor(a0, off, GL);
-                a1 += __shfl_xor(a1, off, GL);
-            }
+            a0 = xor_sum<GL>(a0);
+            a1 = xor_sum<GL>(a1);
             if (lane == 0) {
                 x[2 * j] -= a0;
                 x[2 * j + 1] -= a1;
@@ -390,11 +387,8 @@ __device__ __forceinline__ void wg_solve(const DevSym &Y, const T *blk, double *
                 a0 += (double)U.a * x[2 * r] + (double)U.b * x[2 * r + 1];
                 a1 += (double)U.c * x[2 * r] + (double)U.d * x[2 * r + 1];
             }
-#pragma unroll
-            for (int off = GL / 2; off > 0; off >>= 1) {
-                a0 += __shfl_xor(a0, off, GL);
-                a1 += __shfl_xor(a1, off, GL);
-            }
+            a0 = xor_sum<GL>(a0);
+            a1 = xor_sum<GL>(a1);
             if (lane == 0) {
                 const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
                 const B4<T> D = (i == grp) ? pD : load_b4(blk, Y.di0 + j);
@@ -508,11 +502,8 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                     a0 += (double)L.a * x0 + (double)L.b * x1;
                     a1 += (double)L.c * x0 + (double)L.d * x1;
                 }
-#pragma unroll
-                for (int off = GL / 2; off > 0; off >>= 1) {
-                    a0 += __shfl_xor(a0, off, GL);
-                    a1 += __shfl_xor(a1, off, GL);
-                }
+                a0 = xor_sum<GL>(a0);
+                a1 = xor_sum<GL>(a1);
                 if (lane == 0) {
                     x[2 * j] -= a0;
                     x[2 * j + 1] -= a1;
@@ -583,11 +574,8 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                     a0 += (double)U.a * x0 + (double)U.b * x1;
                     a1 += (double)U.c * x0 + (double)U.d * x1;
                 }
-#pragma unroll
-                for (int off = GL / 2; off > 0; off >>= 1) {
-                    a0 += __shfl_xor(a0, off, GL);
-                    a1 += __shfl_xor(a1, off, GL);
-                }
+                a0 = xor_sum<GL>(a0);
+                a1 = xor_sum<GL>(a1);
                 if (lane == 0) {
                     const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
                     x[2 * j] = (double)myD.a * y0 + (double)myD.b * y1;
@@ -681,10 +669,8 @@ __device__ void wg_pq(const DevNet &N, Work<T> &w) {
         }
 #pragma unroll
         for (int u = 0; u < 2; u++) {
-            ar[u] += __shfl_xor(ar[u], 1, 4);
-            ai[u] += __shfl_xor(ai[u], 1, 4);
-            ar[u] += __shfl_xor(ar[u], 2, 4);
-            ai[u] += __shfl_xor(ai[u], 2, 4);
+            ar[u] = quad_sum(ar[u]);
+            ai[u] = quad_sum(ai[u]);
             if (row[u] < N.nB && lane == 0) {
                 const double2 Ek = w.E[row[u]];
                 w.I[row[u]] = make_double2(ar[u], ai[u]);
@@ -733,10 +719,8 @@ __device__ void wg_residual(const DevNet &N, Work<T> &w) {
         }
 #pragma unroll
         for (int u = 0; u < 2; u++) {
-            ar[u] += __shfl_xor(ar[u], 1, 4);
-            ai[u] += __shfl_xor(ai[u], 1, 4);
-            ar[u] += __shfl_xor(ar[u], 2, 4);
-            ai[u] += __shfl_xor(ai[u], 2, 4);
+            ar[u] = quad_sum(ar[u]);
+            ai[u] = quad_sum(ai[u]);
             if (active[u] && lane == 0) {
                 const int k = row[u], j = N.node_of_bus[k];
                 const double dth = w.x[2 * j];

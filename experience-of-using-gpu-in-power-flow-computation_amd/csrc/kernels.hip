@@ -92,11 +92,8 @@ __global__ void __launch_bounds__(256) k_pq(DevNet N, DevState S, int gate) {
             ai += y.x * Em.y + y.y * Em.x;
         }
     }
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) {
-        ar += __shfl_xor(ar, off, LPR);
-        ai += __shfl_xor(ai, off, LPR);
-    }
+    ar = xor_sum<LPR>(ar);
+    ai = xor_sum<LPR>(ai);
     if (row < N.nB && lane == 0) {
         double2 Ek = win[row - lo];
         S.I[row] = make_double2(ar, ai);
@@ -423,11 +420,8 @@ __device__ __forceinline__ void fwd_node(const DevSym &Y, const T *blk, double *
         a0 += (double)L.a * x0 + (double)L.b * x1;
         a1 += (double)L.c * x0 + (double)L.d * x1;
     }
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
-        a0 += __shfl_xor(a0, off, 16);
-        a1 += __shfl_xor(a1, off, 16);
-    }
+    a0 = xor_sum<16>(a0);
+    a1 = xor_sum<16>(a1);
     if (sl == 0) {
         x[2 * j] -= a0;
         x[2 * j + 1] -= a1;
@@ -445,11 +439,8 @@ __device__ __forceinline__ void bwd_node(const DevSym &Y, const T *blk, double *
         a0 += (double)U.a * x0 + (double)U.b * x1;
         a1 += (double)U.c * x0 + (double)U.d * x1;
     }
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
-        a0 += __shfl_xor(a0, off, 16);
-        a1 += __shfl_xor(a1, off, 16);
-    }
+    a0 = xor_sum<16>(a0);
+    a1 = xor_sum<16>(a1);
     if (sl == 0) {
         const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
         B4<T> D = load_b4(blk, Y.di0 + j);
@@ -647,11 +638,8 @@ __global__ void __launch_bounds__(1024) k_bwd_tail(DevSym Y, const T *blk, doubl
 // sweep) are summed first by a grid-wide pass, 16 lanes per tail node.
 
 __device__ __forceinline__ void sum16(double &a0, double &a1) {
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) {
-        a0 += __shfl_xor(a0, off, 16);
-        a1 += __shfl_xor(a1, off, 16);
-    }
+    a0 = xor_sum<16>(a0);
+    a1 = xor_sum<16>(a1);
 }
 
 // forward rows of the tail nodes: x_j -= sum over entries on nodes < t0
@@ -1005,11 +993,8 @@ __global__ void __launch_bounds__(256) k_residual(DevNet N, DevState S, int gate
             ai += y.x * di + y.y * dr;
         }
     }
-#pragma unroll
-    for (int off = LPR / 2; off > 0; off >>= 1) {
-        ar += __shfl_xor(ar, off, LPR);
-        ai += __shfl_xor(ai, off, LPR);
-    }
+    ar = xor_sum<LPR>(ar);
+    ai = xor_sum<LPR>(ai);
     if (active && lane == 0) {
         const int j = N.node_of_bus[row];
         const double dth = S.x[2 * j];

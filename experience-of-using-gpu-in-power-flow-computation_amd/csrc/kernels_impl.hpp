@@ -53,16 +53,52 @@ __device__ __forceinline__ void mma2(B4<T> &c, const B4<T> &a, const B4<T> &b) {
     c.d += a.c * b.b + a.d * b.d;
 }
 
-template <typename T, int G>
-__device__ __forceinline__ B4<T> group_sum(B4<T> v) {
+// v + v of the lane that the DPP control selects within the row of 16 lanes
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return v + __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The xor butterfly over aligned groups of W lanes,
+//     for (off = W / 2; off > 0; off /= 2) v += v of lane (lane ^ off),
+// every lane ending with the group's sum.  For W = 2, 4 and 16 the partner
+// values come from DPP moves (no LDS round trip): offsets 2 and 1 are quad
+// permutations, and 8 and 4 are row rotations, which read a lane holding
+// the same bits as lane ^ off because each step leaves the values periodic
+// in the offsets already taken.  Other widths use ds_bpermute.
+template <int W, typename T>
+__device__ __forceinline__ T xor_sum(T v) {
+    if constexpr (W == 2 || W == 4 || W == 16) {
+        if constexpr (W == 16) {
+            v = dpp_add<0x128>(v);   // row_ror:8  = lane ^ 8
+            v = dpp_add<0x124>(v);   // row_ror:4 ~ lane ^ 4 (period 8 now)
+        }
+        if constexpr (W >= 4) v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1] = lane ^ 2
+        v = dpp_add<0xB1>(v);                          // quad_perm [1,0,3,2] = lane ^ 1
+    } else {
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        v.a += __shfl_xor(v.a, off, G);
-        v.b += __shfl_xor(v.b, off, G);
-        v.c += __shfl_xor(v.c, off, G);
-        v.d += __shfl_xor(v.d, off, G);
+        for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
     }
     return v;
+}
+
+// The sum of an aligned quad of lanes as (p0 + p1) + (p2 + p3), in every
+// lane of the quad.
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v) {
+    return dpp_add<0x4E>(dpp_add<0xB1>(v));
+}
+
+template <typename T, int G>
+__device__ __forceinline__ B4<T> group_sum(B4<T> v) {
+    return B4<T>{xor_sum<G>(v.a), xor_sum<G>(v.b), xor_sum<G>(v.c), xor_sum<G>(v.d)};
 }
 
 // Sums the update pairs q, q + step, ... < qe of one destination block:
@@ -335,19 +371,6 @@ __device__ __forceinline__ B4<T> lds_sum_in_order(const T *v, int lo, int hi) {
         a = B4<T>{a.a + x.a, a.b + x.b, a.c + x.c, a.d + x.d};
     }
     return a;
-}
-
-// v + v of the lane that the DPP control selects within the row of 16 lanes
-template <int CTRL>
-__device__ __forceinline__ float dpp_add(float v) {
-    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_add(double v) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
-    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
-    return v + __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Sum of an aligned group of g = 4 or 16 lanes' values p_0 .. p_{g-1} as the
@@ -801,11 +824,8 @@ __device__ void chain_solve(const DevSym &Y, const T *blk, double *x, double *s_
             a0 += (double)L.a * x0 + (double)L.b * x1;
             a1 += (double)L.c * x0 + (double)L.d * x1;
         }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            a0 += __shfl_xor(a0, off, G);
-            a1 += __shfl_xor(a1, off, G);
-        }
+        a0 = xor_sum<G>(a0);
+        a1 = xor_sum<G>(a1);
         if (lane == 0) {
             s_r[2 * a] = x[2 * c] - a0;
             s_r[2 * a + 1] = x[2 * c + 1] - a1;
@@ -820,11 +840,8 @@ __device__ void chain_solve(const DevSym &Y, const T *blk, double *x, double *s_
             a0 += (double)S.a * r0 + (double)S.b * r1;
             a1 += (double)S.c * r0 + (double)S.d * r1;
         }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            a0 += __shfl_xor(a0, off, G);
-            a1 += __shfl_xor(a1, off, G);
-        }
+        a0 = xor_sum<G>(a0);
+        a1 = xor_sum<G>(a1);
         if (lane == 0) {
             x[2 * (c0 + a)] = a0;
             x[2 * (c0 + a) + 1] = a1;
@@ -859,9 +876,9 @@ __device__ __forceinline__ double u64_as_double(unsigned long long v) {
 __device__ __forceinline__ double dmax_nan(double a, double b) { return (a != a || a > b) ? a : b; }
 
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+    v += __shfl_xor(v, 32);
+    v += __shfl_xor(v, 16);
+    return xor_sum<16>(v);   // the butterfly's offsets 8 .. 1 (period 16 from here)
 }
 
 __device__ __forceinline__ double wave_max(double v) {

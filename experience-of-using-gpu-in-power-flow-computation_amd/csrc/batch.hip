@@ -110,7 +110,7 @@ __host__ __device__ inline long long work_bytes(int nB, int n, int nPV, int nblk
 }
 
 template <typename T>
-__device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nblk) {
+__device__ __forceinline__ Work<T> carve(char *base, int nB, int n, int nPV, int nblk) {
     Work<T> w;
     char *p = base;
     auto take = [&](size_t bytes) { char *q = p; p += bytes; return q; };
@@ -132,7 +132,7 @@ __device__ inline Work<T> carve(char *base, int nB, int n, int nPV, int nblk) {
 }
 
 template <typename T>
-__device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *bad) {
+__device__ __forceinline__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *bad) {
     const int d0 = Y.fd_ptr[j], nd = Y.fd_ptr[j + 1] - d0;
     const int L0 = Y.n, U0 = Y.n + Y.nnzL;
     B4<T> dinv{0, 0, 0, 0};
@@ -193,7 +193,7 @@ __device__ void wg_factor_node(const DevSym &Y, T *blk, int j, int lane, int *ba
 // node of the level) and the first updated column block stays in registers,
 // so the second half of a level waits on no global load.
 template <typename T, int G>
-__device__ void wg_factor_narrow(const DevSym &Y, T *blk, int l0, int l1, int *bad, B4<T> *s_piv) {
+__device__ __forceinline__ void wg_factor_narrow(const DevSym &Y, T *blk, int l0, int l1, int *bad, B4<T> *s_piv) {
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int grp = (int)threadIdx.x / G, lane = (int)threadIdx.x % G, ng = (int)blockDim.x / G;
     struct Dest {
@@ -640,7 +640,7 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
 
 // I = Y E and S = E conj(I): 4 lanes per row
 template <typename T>
-__device__ void wg_pq(const DevNet &N, Work<T> &w) {
+__device__ __forceinline__ void wg_pq(const DevNet &N, Work<T> &w) {
     // two rows a lane group at a time (r and r + rows_per), their entries
     // walked together so that their loads are in flight together; each row
     // sums in the same order as alone
@@ -683,7 +683,7 @@ __device__ void wg_pq(const DevNet &N, Work<T> &w) {
 
 // r = g - J x (FP64, complex voltages)
 template <typename T>
-__device__ void wg_residual(const DevNet &N, Work<T> &w) {
+__device__ __forceinline__ void wg_residual(const DevNet &N, Work<T> &w) {
     // two rows a lane group at a time, as in wg_pq
     const int rows_per = (int)blockDim.x / 4;
     const int lane = (int)threadIdx.x & 3;
@@ -740,7 +740,7 @@ __device__ void wg_residual(const DevNet &N, Work<T> &w) {
 
 // limits pass: mode 0 chkLimQ (or undo when !chk), flips accumulated in *flips
 template <typename T>
-__device__ void wg_limits(const DevNet &N, Work<T> &w, const double *Qliq, const double *Qload,
+__device__ __forceinline__ void wg_limits(const DevNet &N, Work<T> &w, const double *Qliq, const double *Qload,
                           bool chk, int *flips) {
     int f = 0;
     for (int i = threadIdx.x; i < N.nPV; i += blockDim.x) {
@@ -769,7 +769,7 @@ __device__ void wg_limits(const DevNet &N, Work<T> &w, const double *Qliq, const
 }
 
 template <typename T>
-__device__ void wg_set_E(const DevNet &N, Work<T> &w) {
+__device__ __forceinline__ void wg_set_E(const DevNet &N, Work<T> &w) {
     for (int k = threadIdx.x; k < N.nB; k += blockDim.x) {
         double s, c;
         sincos(w.th[k], &s, &c);
@@ -1479,6 +1479,8 @@ pf_status pf_batch::configure(BatchNet &nb, int mode, int block, int per_cu) {
     if (mode == 1 && (nb.lds || nb.partial == 0)) return refuse("mode 1 is for working sets beyond LDS whose step vectors fit");
     if (mode == 0 && nb.lds) return refuse("the working set fits in LDS: use mode 2");
     if (block < 64 || block > 1024 || block % 64) return refuse("block must be a multiple of 64 up to 1024");
+    // the sweeps of modes 1 and 0 give each node of a narrow level its own 16 lanes
+    if (mode != 2 && block < GL * kSweepGroupLevel) return refuse("modes 1 and 0 take at least 256 threads");
     int occ = 0;
     BCHK(occupancy(nb, mode, block, &occ));
     if (occ < 1) return refuse("no workgroup fits on a CU (registers or LDS)");

@@ -172,7 +172,7 @@ __device__ __forceinline__ void pairs_acc(const DevSym &Y, const T *blk, int q, 
 // and of each lane's first pair on the next level do not depend on values and
 // are loaded one level ahead.
 template <typename T, int G, int UNR>
-__device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *bad) {
+__device__ __forceinline__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *bad) {
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int grp = (int)threadIdx.x / G, lane = (int)threadIdx.x % G, ng = (int)blockDim.x / G;
     struct Dest {
@@ -262,7 +262,7 @@ __device__ void coop_factor_range(const DevSym &Y, T *blk, int l0, int l1, int *
 // loaded before A, and C takes the pivot from LDS and the updated column
 // block from B's registers, so only A waits on global loads.
 template <typename T, int G, int UNR>
-__device__ void chunk_factor_levels(const DevSym &Y, T *blk, int lv0, int lv1, int *bad, B4<T> *s_pa,
+__device__ __forceinline__ void chunk_factor_levels(const DevSym &Y, T *blk, int lv0, int lv1, int *bad, B4<T> *s_pa,
                                     B4<T> *s_pu, B4<T> *s_piv) {
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x, grp = tid / G, lane = tid % G, ng = bd / G;
@@ -597,7 +597,7 @@ __device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFla
 // one or many workgroups can share the pass); every destination is
 // independent.  The caller orders it after every node before the chain.
 template <typename T, int G, int UNR>
-__device__ void chain_schur(const DevSym &Y, T *blk, int group0, int ngroups) {
+__device__ __forceinline__ void chain_schur(const DevSym &Y, T *blk, int group0, int ngroups) {
     const int m = Y.ch_m, c0 = Y.ch_c0;
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x;
@@ -668,7 +668,7 @@ __device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B
 // updated but never stored).  Both forms give the same values.
 // The caller orders it after chain_schur and synchronises after.
 template <typename T, int TR, int TC>
-__device__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
+__device__ __forceinline__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
     const int m = Y.ch_m, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x;
     const int ntc = (m + TC - 1) / TC, ntr = (m + TR - 1) / TR;
@@ -774,7 +774,7 @@ __device__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
 // updated where it is stored (LDS or a global slice), two barriers per
 // step.  Every variant does the same arithmetic.
 template <typename T, int G, int UNR, int TMAX>
-__device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
+__device__ __forceinline__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
     const int m = Y.ch_m, mm = m * m, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
     for (int q = tid; q < mm; q += bd) store_b4(blk, SB + q, B4<T>{0, 0, 0, 0});   // outside L: zero
@@ -809,7 +809,7 @@ __device__ void chain_factor_mem(const DevSym &Y, T *blk, int *bad, B4<T> *lds) 
 // side (s_r: 2 ch_m doubles of LDS).  The backward sweep then starts below
 // the chain.  Two barriers.
 template <typename T, int G>
-__device__ void chain_solve(const DevSym &Y, const T *blk, double *x, double *s_r) {
+__device__ __forceinline__ void chain_solve(const DevSym &Y, const T *blk, double *x, double *s_r) {
     const int m = Y.ch_m, c0 = Y.ch_c0;
     const int LR0 = Y.n + 2 * Y.nnzL, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x, grp = tid / G, lane = tid % G, ng = (int)blockDim.x / G;

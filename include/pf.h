@@ -307,7 +307,8 @@ pf_status pf_batch_solve_noalloc(pf_batch *b, int n_scen, const int *net_of, con
 pf_status pf_batch_autotune(pf_batch *b, const char *cache_path, int reps, int *n_measured);
 /* fix the launch shape of network `net` (after pf_batch_set_scenarios):
  * mode 2 needs the working set to fit in LDS, 1 the step vectors and level
- * metadata, 0 always fits; block a multiple of 64 up to 1024; per_cu <= 0:
+ * metadata, 0 always fits; block a multiple of 64 up to 1024 (modes 1 and
+ * 0: at least 256); per_cu <= 0:
  * as many workgroups per CU as fit.  PF_INVALID_ARG when it cannot run.     */
 pf_status pf_batch_set_shape(pf_batch *b, int net, int mode, int block, int per_cu);
 /* launch shape in use for network `net`: kernel mode (2 LDS, 1 partial LDS,

@@ -323,6 +323,32 @@ def test_batch_1888(pf, oracle, synth, cases, precision):
             assert row[3] == row[4] and row[5] == row[6], row
 
 
+def test_batch_global_modes_workgroup_sizes(pf, synth, cases):
+    """The placements outside LDS (modes 1 and 0) take at least 256
+    threads: their sweeps give each node of a narrow level (up to 16 nodes)
+    its own 16 lanes.  Smaller workgroups are refused; the sizes that run
+    give the same bits."""
+    n944 = pf.Network(cases.path("c944"))
+    P, Q, L = _scenarios(synth, n944, 3, 3700000)
+    b = pf.Batch([n944], pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=1, precision=pf.MIXED))
+    b.set_scenarios(np.zeros(3, np.int32), P.ravel(), Q.ravel(), L.ravel())
+    for mode in (1, 0):
+        for block in (64, 128, 192):
+            with pytest.raises(pf.PfError):
+                b.set_shape(0, mode, block)
+    outs = []
+    for mode in (1, 0):
+        for block in (256, 320, 512, 1024):
+            b.set_shape(0, mode, block)
+            _say("c944 mode", mode, "block", block)
+            b.solve()
+            outs.append(b.results())
+    assert outs[0][3].all()
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_odd_size_network(pf, oracle, synth, precision):
     """A network with an odd bus count (IEEE 57-bus): the per-scenario working

@@ -1017,7 +1017,7 @@ template <typename T, int MODE, int MAXT>   // MAXT: largest workgroup (register
 __global__ void __launch_bounds__(MAXT, (batch_min_waves<T, MODE, MAXT>())) k_batch(BatchTopo tp0, BatchJob J) {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_item, s_it, s_conv, s_fl;
-    __shared__ B4<T> s_chain[4 * batch_chain_cap<MODE>()];   // dense top: row / column staging
+    __shared__ B4<T> s_chain[4 * batch_chain_cap<MODE>() + 2];   // dense top: row / column / pivot staging
     __shared__ B4<T> s_ck[MODE == 2 ? 1 : 2 * kBatchMaxChunks];  // chunk partial sums (a, u)
     __shared__ double s_maxg;
     BatchTopo tp = tp0;

@@ -404,7 +404,7 @@ __global__ void __launch_bounds__(256) k_chain_schur(DevSym Y, T *blk, Ctl *ctl,
 template <typename T>
 __global__ void __launch_bounds__(ChainTiles<T>::threads) k_chain_invert(DevSym Y, T *blk, Ctl *ctl, int gate) {
     if (gate && !ctl->step) return;
-    __shared__ B4<T> lds[4 * kChainMax];
+    __shared__ B4<T> lds[4 * kChainMax + 2];
     chain_invert<T, ChainTiles<T>::tr, ChainTiles<T>::tc>(Y, blk, &ctl->singular, lds);
 }
 

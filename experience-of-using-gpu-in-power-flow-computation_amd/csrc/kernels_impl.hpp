@@ -639,6 +639,17 @@ __device__ __forceinline__ B4<T> pivot_inverse(const B4<T> &pv, int *bad) {
     return B4<T>{pv.d / det, -pv.b / det, -pv.c / det, pv.a / det};
 }
 
+// 2x2 pivot inverse formed by one thread, which raises *bad itself
+template <typename T>
+__device__ __forceinline__ B4<T> pivot_inverse_own(const B4<T> &pv, int *bad) {
+    T det = pv.a * pv.d - pv.b * pv.c;
+    if (!(det != (T)0) || !isfinite((double)det)) {
+        atomicOr(bad, 1);
+        det = (T)1;
+    }
+    return B4<T>{pv.d / det, -pv.b / det, -pv.c / det, pv.a / det};
+}
+
 // One Gauss-Jordan update of block (i, j) at step k, from the step's pivot
 // inverse P and the old row k (rk = S_kj) and column k (ci = S_ik):
 //   S_kk <- P; S_kj <- P S_kj; S_ik <- -S_ik P; S_ij <- S_ij - S_ik (P S_kj).
@@ -659,8 +670,11 @@ __device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B
 // during the elimination, as TR x TC tiles of blocks: thread t owns rows
 // (t / ntc) * TR + a and columns (t % ntc) * TC + c, with ntc = ceil(m / TC)
 // (requires ceil(m / TR) * ceil(m / TC) <= blockDim).  Row k and column k go
-// through LDS (lds: 4 ch_m blocks, double buffered: one barrier per step); a
-// thread reads TR + TC blocks of them per step, not two per block it owns.
+// through LDS (lds: 4 ch_m + 2 blocks, double buffered: one barrier per
+// step); a thread reads TR + TC blocks of them per step, not two per block it
+// owns.  The thread holding the pivot block inverts it before the barrier
+// and leaves the inverse in LDS beside the row, so the other threads do not
+// each invert it (the update is bound by arithmetic, not by the barrier).
 // In FP32 every block of a tile takes the general update
 // S_ij -= S_ik (P S_kj) without a test; the few threads holding row k or
 // column k then overwrite those blocks with P S_kj, -S_ik P and P, so the
@@ -684,15 +698,17 @@ __device__ __forceinline__ void chain_invert(const DevSym &Y, T *blk, int *bad, 
     auto jc = [&](int c) { return min(j0 + c, m - 1); };
     auto ic = [&](int a) { return min(i0 + a, m - 1); };
     for (int k = 0; k < m; k++) {
-        B4<T> *s_row = lds + (k & 1) * 2 * m, *s_col = s_row + m;
+        B4<T> *s_row = lds + (k & 1) * (2 * m + 1), *s_col = s_row + m, *s_p = s_col + m;
         const int ar = k - i0, ac = k - j0;   // pivot row / column inside this tile when in [0, TR) / [0, TC)
         if (k >= i0 && k < i0 + TR) {   // this thread holds part of row k
 #pragma unroll
             for (int a = 0; a < TR; a++)
                 if (i0 + a == k) {
 #pragma unroll
-                    for (int c = 0; c < TC; c++)
+                    for (int c = 0; c < TC; c++) {
                         if (j0 + c < m) s_row[j0 + c] = reg[a][c];
+                        if (j0 + c == k) *s_p = pivot_inverse_own(reg[a][c], bad);
+                    }
                 }
         }
         if (k >= j0 && k < j0 + TC) {   // ... of column k
@@ -705,8 +721,8 @@ __device__ __forceinline__ void chain_invert(const DevSym &Y, T *blk, int *bad, 
                 }
         }
         __syncthreads();
-        const B4<T> P = pivot_inverse(s_row[k], bad);
         if (!own) continue;
+        const B4<T> P = *s_p;
         if constexpr (sizeof(T) == 4) {   // registers to spare: read row and column once
             B4<T> pr[TC], ci[TR];
 #pragma unroll
@@ -769,7 +785,7 @@ __device__ __forceinline__ void chain_invert(const DevSym &Y, T *blk, int *bad, 
 }
 
 // Schur complement and inverse of the dense top block in one workgroup (lds:
-// 4 ch_m blocks).  S sits in registers in the smallest square tiles of at
+// 4 ch_m + 2 blocks).  S sits in registers in the smallest square tiles of at
 // most TMAX x TMAX blocks that the workgroup covers; when none does, S is
 // updated where it is stored (LDS or a global slice), two barriers per
 // step.  Every variant does the same arithmetic.

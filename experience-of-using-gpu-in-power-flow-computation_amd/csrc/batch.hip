@@ -443,16 +443,23 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
     constexpr int RA = kSweepAhead;
     int pk[RA];
     B4<T> pB[RA], pD{0, 0, 0, 0};
+    int ns = 0, nc = 0, ne0 = 0, ne1 = 0;   // the coming level's first node and width, this lane's node's entries
     // the entries this lane takes first on level l: a narrow level's node
     // grp (entries lane + GL r), a wide level's entry tid; ptr / idx / B0 are
-    // the row (forward) or column (backward) structure and its blocks
+    // the row (forward) or column (backward) structure and its blocks.  The
+    // level's bounds and the node's entry range stay in registers for the
+    // level itself, which then starts with no metadata read.
     auto ahead = [&](int l, const int *ptr, const int *idx, int B0, bool diag) __attribute__((always_inline)) {
         const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        ns = s;
+        nc = c;
 #pragma unroll
         for (int r = 0; r < RA; r++) pk[r] = -1;
         if (c <= kSweepGroupLevel) {
             if (grp < c) {
                 const int j = s + grp, e1 = ptr[j + 1];
+                ne0 = ptr[j];
+                ne1 = e1;
 #pragma unroll
                 for (int r = 0; r < RA; r++) {
                     const int e = ptr[j] + lane + GL * r;
@@ -475,7 +482,7 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
     // ---- forward
     if (nl > 0) ahead(0, Y.rs_ptr, Y.rs_k, LR0, false);
     for (int l = 0; l < nl; l++) {
-        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        const int s = ns, c = nc, je0 = ne0, je1 = ne1;
         int ck[RA];
         B4<T> cB[RA];
 #pragma unroll
@@ -484,6 +491,13 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
             cB[r] = pB[r];
         }
         if (c <= kSweepGroupLevel) {
+            double xk[RA][2];   // x of the first entries, read before the next level's requests
+#pragma unroll
+            for (int r = 0; r < RA; r++) {
+                const int k = ck[r] >= 0 ? ck[r] : 0;
+                xk[r][0] = x[2 * k];
+                xk[r][1] = x[2 * k + 1];
+            }
             if (l + 1 < nl) ahead(l + 1, Y.rs_ptr, Y.rs_k, LR0, false);
             if (grp < c) {
                 const int j = s + grp;
@@ -491,11 +505,11 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
 #pragma unroll
                 for (int r = 0; r < RA; r++)
                     if (ck[r] >= 0) {
-                        const double x0 = x[2 * ck[r]], x1 = x[2 * ck[r] + 1];
+                        const double x0 = xk[r][0], x1 = xk[r][1];
                         a0 += (double)cB[r].a * x0 + (double)cB[r].b * x1;
                         a1 += (double)cB[r].c * x0 + (double)cB[r].d * x1;
                     }
-                for (int e = Y.rs_ptr[j] + lane + GL * RA; e < Y.rs_ptr[j + 1]; e += GL) {
+                for (int e = je0 + lane + GL * RA; e < je1; e += GL) {
                     const B4<T> L = load_b4(blk, LR0 + e);
                     const int k = Y.rs_k[e];
                     const double x0 = x[2 * k], x1 = x[2 * k + 1];
@@ -546,7 +560,7 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
     // ---- backward
     if (nl > 0) ahead(nl - 1, Y.colptr, Y.rowind, U0, true);
     for (int l = nl - 1; l >= 0; l--) {
-        const int s = Y.lvl_ptr[l], c = Y.lvl_ptr[l + 1] - s;
+        const int s = ns, c = nc, je0 = ne0, je1 = ne1;
         int ck[RA];
         B4<T> cB[RA];
 #pragma unroll
@@ -556,6 +570,13 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
         }
         const B4<T> myD = pD;   // narrow: the inverse pivot of node s + grp (lane 0); wide: of node s + tid
         if (c <= kSweepGroupLevel) {
+            double xk[RA][2];   // x of the first entries, read before the next level's requests
+#pragma unroll
+            for (int r = 0; r < RA; r++) {
+                const int k = ck[r] >= 0 ? ck[r] : 0;
+                xk[r][0] = x[2 * k];
+                xk[r][1] = x[2 * k + 1];
+            }
             if (l > 0) ahead(l - 1, Y.colptr, Y.rowind, U0, true);
             if (grp < c) {
                 const int j = s + grp;
@@ -563,11 +584,11 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
 #pragma unroll
                 for (int r = 0; r < RA; r++)
                     if (ck[r] >= 0) {
-                        const double x0 = x[2 * ck[r]], x1 = x[2 * ck[r] + 1];
+                        const double x0 = xk[r][0], x1 = xk[r][1];
                         a0 += (double)cB[r].a * x0 + (double)cB[r].b * x1;
                         a1 += (double)cB[r].c * x0 + (double)cB[r].d * x1;
                     }
-                for (int e = Y.colptr[j] + lane + GL * RA; e < Y.colptr[j + 1]; e += GL) {
+                for (int e = je0 + lane + GL * RA; e < je1; e += GL) {
                     const B4<T> U = load_b4(blk, U0 + e);
                     const int i = Y.rowind[e];
                     const double x0 = x[2 * i], x1 = x[2 * i + 1];

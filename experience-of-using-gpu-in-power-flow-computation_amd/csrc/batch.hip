@@ -498,6 +498,9 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 xk[r][0] = x[2 * k];
                 xk[r][1] = x[2 * k + 1];
             }
+            // the node's own x: no earlier level of this sweep writes it
+            const int jo = s + (grp < c ? grp : 0);
+            const double xj0 = x[2 * jo], xj1 = x[2 * jo + 1];
             if (l + 1 < nl) ahead(l + 1, Y.rs_ptr, Y.rs_k, LR0, false);
             if (grp < c) {
                 const int j = s + grp;
@@ -519,8 +522,8 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 a0 = xor_sum<GL>(a0);
                 a1 = xor_sum<GL>(a1);
                 if (lane == 0) {
-                    x[2 * j] -= a0;
-                    x[2 * j + 1] -= a1;
+                    x[2 * j] = xj0 - a0;
+                    x[2 * j + 1] = xj1 - a1;
                 }
             }
             __syncthreads();
@@ -577,6 +580,9 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 xk[r][0] = x[2 * k];
                 xk[r][1] = x[2 * k + 1];
             }
+            // the node's own x: no earlier level of this sweep writes it
+            const int jo = s + (grp < c ? grp : 0);
+            const double xj0 = x[2 * jo], xj1 = x[2 * jo + 1];
             if (l > 0) ahead(l - 1, Y.colptr, Y.rowind, U0, true);
             if (grp < c) {
                 const int j = s + grp;
@@ -598,7 +604,7 @@ __device__ __forceinline__ void wg_solve_flat(const DevSym &Y, const T *blk, dou
                 a0 = xor_sum<GL>(a0);
                 a1 = xor_sum<GL>(a1);
                 if (lane == 0) {
-                    const double y0 = x[2 * j] - a0, y1 = x[2 * j + 1] - a1;
+                    const double y0 = xj0 - a0, y1 = xj1 - a1;
                     x[2 * j] = (double)myD.a * y0 + (double)myD.b * y1;
                     x[2 * j + 1] = (double)myD.c * y0 + (double)myD.d * y1;
                 }

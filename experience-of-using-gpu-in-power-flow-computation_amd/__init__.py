@@ -117,6 +117,8 @@ def lib():
             L.pf_warmstart_fit.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                            C.c_ulonglong, P]
             L.pf_warmstart_predict.argtypes = [P, P, C.c_int, P]
+            L.pf_warmstart_stage.argtypes = [P, P, C.c_int]
+            L.pf_warmstart_predict_into.argtypes = [P, P, C.c_int, C.c_int]
             L.pf_warmstart_gemm_probe.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P]
         if hasattr(L, "pf_batch_create"):
             L.pf_batch_create.argtypes = [P, C.c_int, C.POINTER(Options), C.c_int, C.POINTER(P)]
@@ -514,6 +516,16 @@ class WarmStart:
         Y = np.zeros((X.shape[0], self.n_out), np.float32)
         _chk(lib().pf_warmstart_predict(self.h, _p(X), X.shape[0], _p(Y)), "pf_warmstart_predict")
         return Y
+
+    def stage(self, X):
+        """Features of every scenario of a batch to the GPU, once."""
+        X = np.ascontiguousarray(X, np.float32)
+        _chk(lib().pf_warmstart_stage(self.h, _p(X), X.shape[0]), "pf_warmstart_stage")
+
+    def predict_into(self, batch, v_col: int, th_col: int):
+        """Inference on the staged rows, written into `batch` as each scenario's
+        starting |V| (row columns v_col..) and theta (th_col..)."""
+        _chk(lib().pf_warmstart_predict_into(self.h, batch.h, v_col, th_col), "pf_warmstart_predict_into")
 
     def gemm_probe(self, A, B, ta=False, tb=False):
         A = np.ascontiguousarray(A, np.float32)

@@ -1210,6 +1210,20 @@ inline BatchKernel batch_kernel(bool mixed, int mode, int block) {
                      : mode == 1 ? batch_kernel_t<double, 1>(block) : batch_kernel_t<double, 0>(block);
 }
 
+// Starting state of scenario s (one workgroup each) from row s of a device
+// array of network outputs, de-normalised as the inference's outputs are.
+__global__ void k_start_from_rows(const float *Y, long long ld, int v_col, int th_col, const float *mu,
+                                  const float *sd, const long long *off, int n_scen, long long total, double *V0,
+                                  double *T0) {
+    const int s = (int)blockIdx.x;
+    const long long o = off[s], nb = (s + 1 < n_scen ? off[s + 1] : total) - o;
+    const float *row = Y + (long long)s * ld;
+    for (long long i = threadIdx.x; i < nb; i += blockDim.x) {
+        V0[o + i] = (double)(row[v_col + i] * sd[v_col + i] + mu[v_col + i]);
+        T0[o + i] = (double)(row[th_col + i] * sd[th_col + i] + mu[th_col + i]);
+    }
+}
+
 }  // namespace pf
 
 // ======================================================================== host
@@ -1575,6 +1589,41 @@ pf_status pf_batch::launch_all() {
     for (size_t r = 0; r < order.size(); r++) BCHK(launch_net(order[r], st[r & 1]));
     BHIP(hipEventRecord(ev_join, st[1]));
     BHIP(hipStreamWaitEvent(st[0], ev_join, 0));
+    return PF_OK;
+}
+
+pf_status pf::batch_start_from_rows(pf_batch *b, int device, int rows, const float *Y, long long ld, int v_col,
+                                    int th_col, const float *mu, const float *sd, void *stream) {
+    if (!b || !b->n_scen || !Y || !mu || !sd || v_col < 0 || th_col < 0) return PF_INVALID_ARG;
+    if (device != b->device) {
+        pf::set_last_error("warm start: the network and the batch are on different GPUs");
+        return PF_INVALID_ARG;
+    }
+    if (rows != b->n_scen) {
+        pf::set_last_error("warm start: one row per batch scenario");
+        return PF_INVALID_ARG;
+    }
+    int nbmax = 0;
+    for (int s : b->net_of) nbmax = std::max(nbmax, b->nets[s]->net.nB);
+    if ((long long)v_col + nbmax > ld || (long long)th_col + nbmax > ld) {
+        pf::set_last_error("warm start: a row holds fewer outputs than a scenario's buses");
+        return PF_INVALID_ARG;
+    }
+    BHIP(hipSetDevice(b->device));
+    const size_t tot = (size_t)b->total;
+    if (!b->warm_V0) {
+        BCHK(b->alloc(&b->warm_V0, tot));
+        BCHK(b->alloc(&b->warm_T0, tot));
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    k_start_from_rows<<<b->n_scen, 256, 0, st>>>(Y, ld, v_col, th_col, mu, sd, b->d_off, b->n_scen, b->total,
+                                                b->warm_V0, b->warm_T0);
+    BHIP(hipGetLastError());
+    BHIP(hipStreamSynchronize(st));
+    const bool had_warm = b->d_V0 != nullptr;
+    b->d_V0 = b->warm_V0;
+    b->d_T0 = b->warm_T0;
+    if (!had_warm) b->drop_graph();   // kernel arguments change: capture again
     return PF_OK;
 }
 

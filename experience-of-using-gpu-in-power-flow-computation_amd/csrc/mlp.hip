@@ -234,6 +234,7 @@ struct pf_warmstart {
     size_t n_params = 0;
     int step = 0;
     int Bcap = 0;
+    int staged = 0;   // rows of features on the device for pf_warmstart_predict_into
     hipStream_t stream = nullptr;
     std::vector<void *> allocs, ws_allocs;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
@@ -273,6 +274,7 @@ pf_status pf_warmstart::ensure(int B) {
     if (B <= Bcap) return PF_OK;
     for (void *q : ws_allocs) MHIP(hipFree(q));
     ws_allocs.clear();
+    staged = 0;   // the staged rows went with the old workspace
     Bcap = std::max(B, 64);
     const size_t b = Bcap;
     int maxN = std::max(n_out, hidden);
@@ -482,6 +484,7 @@ pf_status pf_warmstart_set_params(pf_warmstart *w, const float *in) {
 // Y [B x n_out]); gradients (parameter layout) copied to grads if not NULL
 pf_status pf_warmstart_grad(pf_warmstart *w, const float *X, const float *Y, int B, float *grads, double *loss) {
     if (!w || !X || !Y || B <= 0 || B % 8) return PF_INVALID_ARG;
+    w->staged = 0;   // its rows replace the staged ones
     MHIP(hipSetDevice(w->device));
     MCHK(w->ensure(B));
     MHIP(hipMemcpyAsync(w->X, X, sizeof(float) * (size_t)B * w->n_in, hipMemcpyHostToDevice, w->stream));
@@ -499,6 +502,7 @@ pf_status pf_warmstart_grad(pf_warmstart *w, const float *X, const float *Y, int
 pf_status pf_warmstart_fit(pf_warmstart *w, const float *X, const float *Y, int n, int epochs, int B, float lr,
                            float weight_decay, unsigned long long seed, double *epoch_loss) {
     if (!w || !X || !Y || n < B || B <= 0 || B % 8 || epochs <= 0) return PF_INVALID_ARG;
+    w->staged = 0;   // the minibatches replace the staged rows
     MHIP(hipSetDevice(w->device));
     MCHK(w->ensure(B));
     float *dX = nullptr, *dY = nullptr;
@@ -552,6 +556,7 @@ pf_status pf_warmstart_fit(pf_warmstart *w, const float *X, const float *Y, int 
 // inference: X [B x n_in] raw -> Y [B x n_out] in target units
 pf_status pf_warmstart_predict(pf_warmstart *w, const float *X, int B, float *Y) {
     if (!w || !X || !Y || B <= 0 || B % 8) return PF_INVALID_ARG;
+    w->staged = 0;   // its rows replace the staged ones
     MHIP(hipSetDevice(w->device));
     MCHK(w->ensure(B));
     MHIP(hipMemcpyAsync(w->X, X, sizeof(float) * (size_t)B * w->n_in, hipMemcpyHostToDevice, w->stream));
@@ -562,6 +567,24 @@ pf_status pf_warmstart_predict(pf_warmstart *w, const float *X, int B, float *Y)
     MHIP(hipMemcpyAsync(Y, w->Yo, sizeof(float) * ny, hipMemcpyDeviceToHost, w->stream));
     MHIP(hipStreamSynchronize(w->stream));
     return PF_OK;
+}
+
+pf_status pf_warmstart_stage(pf_warmstart *w, const float *X, int B) {
+    if (!w || !X || B <= 0 || B % 8) return PF_INVALID_ARG;
+    MHIP(hipSetDevice(w->device));
+    MCHK(w->ensure(B));
+    MHIP(hipMemcpyAsync(w->X, X, sizeof(float) * (size_t)B * w->n_in, hipMemcpyHostToDevice, w->stream));
+    MHIP(hipStreamSynchronize(w->stream));
+    w->staged = B;
+    return PF_OK;
+}
+
+pf_status pf_warmstart_predict_into(pf_warmstart *w, pf_batch *b, int v_col, int th_col) {
+    if (!w || !b || w->staged <= 0) return PF_INVALID_ARG;
+    MHIP(hipSetDevice(w->device));
+    MCHK(w->forward(w->staged));
+    return batch_start_from_rows(b, w->device, w->staged, w->Yp, w->n_out, v_col, th_col, w->ym, w->ys,
+                                 (void *)w->stream);
 }
 
 // one GEMM through the training kernel (tests): C = op(A) op(B), bf16 in, fp32 out

@@ -176,6 +176,14 @@ struct FlatTail {
 };
 FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below);
 FlatTail flat_parts(const Symbolic &s);
+// Starting state of every scenario of a batch from rows of an array on GPU
+// `device` (row s: scenario s; |V| = row[v_col + i] * sd[v_col + i] +
+// mu[v_col + i], theta likewise from th_col), written on `stream` (a
+// hipStream_t).  Rows have ld floats and must hold v_col + nB and
+// th_col + nB of each scenario's network (checked against ld).
+pf_status batch_start_from_rows(pf_batch *b, int device, int rows, const float *Y, long long ld, int v_col,
+                                int th_col, const float *mu, const float *sd, void *stream);
+
 // lanes per destination of a step with nd destinations of at most maxpd
 // products each
 int flat_group_width(int nd, int maxpd);

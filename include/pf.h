@@ -340,6 +340,17 @@ pf_status pf_warmstart_fit(pf_warmstart *w, const float *X, const float *Y, int 
                            int B, float lr, float weight_decay, unsigned long long seed,
                            double *epoch_loss);
 pf_status pf_warmstart_predict(pf_warmstart *w, const float *X, int B, float *Y);
+/* device-resident inference into a batch: pf_warmstart_stage copies B rows
+ * of features to the GPU once (like pf_batch_set_scenarios, outside a timed
+ * solve); pf_warmstart_predict_into runs the network on them and writes, for
+ * batch scenario s (B must equal the batch's scenario count) and bus i of
+ * its network, |V| = Y[s][v_col + i] and theta = Y[s][th_col + i] as the
+ * scenario's starting state, with no host round trip.  The batch keeps that
+ * start until pf_batch_set_scenarios is called again.  Returns after the
+ * start is written.  A later predict, grad or fit call replaces the staged
+ * rows (predict_into then fails until the next stage).                     */
+pf_status pf_warmstart_stage(pf_warmstart *w, const float *X, int B);
+pf_status pf_warmstart_predict_into(pf_warmstart *w, pf_batch *b, int v_col, int th_col);
 /* one product through the training GEMM kernel: C = op(A) op(B)            */
 pf_status pf_warmstart_gemm_probe(pf_warmstart *w, int ta, int tb, int M, int N, int K,
                                   const float *A, const float *B, float *C);

@@ -111,3 +111,68 @@ def test_warm_start_gains_on_944(pf, synth, cases):
     _say("iterations flat", itf.mean(), "warm", itw.mean(), "ms flat", msf, "warm", msw)
     assert cvf.all() and cvw.all()
     assert itw.mean() < itf.mean()
+
+
+def test_warm_start_predict_into_batch(pf, synth):
+    """Inference on features staged on the GPU, written straight into a
+    batch's starting state (no host round trip), solves every scenario to
+    the same bits as the host path (predict, then the start passed to
+    pf_batch_set_scenarios).  The network is the 118-bus case, a short fit
+    on 512 solutions; a predict call in between invalidates the staged rows."""
+    import os
+    net = pf.Network(os.path.join(os.path.dirname(__file__), "golden", "c118.cdf"))
+    is_pv = np.zeros(net.nB, bool)
+    is_pv[net.pv] = True
+    vset = np.where(is_pv, net.V, 0.0)
+    vset[net.slack] = net.V[net.slack]
+    opts = pf.options(tolerancia=1e-8, no_max_iter=60, lim_inj_reat=0, precision=pf.MIXED)
+    n_in, n_out = 3 * net.nB, 2 * net.nB
+    n_in8, n_out8 = -(-n_in // 8) * 8, -(-n_out // 8) * 8
+
+    def feats(P, Q):
+        X = np.zeros((len(P), n_in8), np.float32)
+        X[:, :n_in] = np.concatenate([P, Q, np.repeat(vset[None, :], len(P), 0)], 1)
+        return X
+
+    P, Q, L = synth.scenario_batch(net.Pg, net.Pload, net.Qg, net.Qload, is_pv, np.arange(6100000, 6100512))[:3]
+    b = pf.Batch([net], opts)
+    b.set_scenarios(np.zeros(len(P), np.int32), P.ravel(), Q.ravel(), L.ravel())
+    b.solve()
+    V, th, it, cv, mm = b.results()
+    assert cv.all()
+    X = feats(P, Q)
+    Y = np.zeros((len(P), n_out8), np.float32)
+    Y[:, :n_out] = np.concatenate([V.reshape(len(P), -1), th.reshape(len(P), -1)], 1)
+    w = pf.WarmStart(n_in8, n_out8, 256, 2, seed=5)
+    w.set_norm(X.mean(0), X.std(0) + 1e-6, Y.mean(0), Y.std(0) + 1e-9)
+    w.fit(X, Y, epochs=4, batch=128, lr=1e-3, weight_decay=1e-5)
+
+    Pt, Qt, Lt = synth.scenario_batch(net.Pg, net.Pload, net.Qg, net.Qload, is_pv, np.arange(6200000, 6200256))[:3]
+    Xt = feats(Pt, Qt)
+    Yt = w.predict(Xt)
+    host = pf.Batch([net], opts)
+    host.set_scenarios(np.zeros(len(Pt), np.int32), Pt.ravel(), Qt.ravel(), Lt.ravel(),
+                       Yt[:, :net.nB].astype(np.float64).ravel(), Yt[:, net.nB:n_out].astype(np.float64).ravel())
+    host.solve()
+    r_host = host.results()
+
+    dev = pf.Batch([net], opts)
+    dev.set_scenarios(np.zeros(len(Pt), np.int32), Pt.ravel(), Qt.ravel(), Lt.ravel())
+    dev.solve()
+    r_flat = dev.results()
+    w.stage(Xt)
+    w.predict_into(dev, 0, net.nB)
+    dev.solve()
+    r_dev = dev.results()
+    _say("warm into batch: iterations flat", r_flat[2].mean(), "warm", r_dev[2].mean())
+    for x, y in zip(r_host, r_dev):
+        assert np.array_equal(x, y)
+    assert r_dev[3].all() and r_dev[2].mean() < r_flat[2].mean()
+    # a predict call replaces the staged rows: predict_into refuses until the next stage
+    w.predict(Xt[:8])
+    with pytest.raises(pf.PfError):
+        w.predict_into(dev, 0, net.nB)
+    # a row narrower than a scenario's buses is refused
+    w.stage(Xt)
+    with pytest.raises(pf.PfError):
+        w.predict_into(dev, 0, n_out8 - net.nB + 1)

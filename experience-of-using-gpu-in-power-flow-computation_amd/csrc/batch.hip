@@ -1446,6 +1446,7 @@ pf_status pf_batch::add_net(const pf_network &src) {
             BCHK(upload((int **)&F.fl_split, ft.fl_split));
             BCHK(upload((int **)&F.fl_a, ft.fl_a));
             BCHK(upload((int **)&F.fl_b, ft.fl_b));
+            BCHK(upload((int **)&F.fl_s, ft.fl_s));
             BCHK(upload((int **)&F.de_ptr, ft.de_ptr));
             BCHK(upload((int **)&F.de_node, ft.de_node));
             BCHK(upload((int **)&F.de_p, ft.de_p));

@@ -370,14 +370,20 @@ __global__ void __launch_bounds__(256) k_tail_factor_ext(DevSym Y, DevFlat F, T 
 }
 
 // Flat tail factorisation, second part: the tail levels in one workgroup
-// (flat_factor_levels), the level's products in dynamic LDS.
+// (flat_factor_levels), the level's products in dynamic LDS, in FP32 also
+// the staged blocks of steps of at most cap destinations after them (the
+// FP64 blocks leave no registers for the requests made a step early; in
+// FP32 two pairs per lane do, three would not).
 template <typename T>
-__global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int nsteps, int gate) {
+__global__ void __launch_bounds__(1024) k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int nsteps, int gate,
+                                                           int cap) {
     if (gate && !ctl->step) return;
     extern __shared__ __align__(16) char smem[];
-    T *s_pa = reinterpret_cast<T *>(smem);
-    flat_factor_levels<T, sizeof(T) == 4 ? 3 : 1>(Y, F, blk, 0, nsteps, &ctl->singular, s_pa,
-                                                                            s_pa + 4 * (size_t)F.max_pairs);
+    T *s_pa = reinterpret_cast<T *>(smem), *s_pu = s_pa + 4 * (size_t)F.max_pairs;
+    if constexpr (sizeof(T) == 4)
+        flat_factor_levels<T, 2>(Y, F, blk, 0, nsteps, &ctl->singular, s_pa, s_pu, s_pu + 4 * (size_t)F.max_col, cap);
+    else
+        flat_factor_levels<T, 1>(Y, F, blk, 0, nsteps, &ctl->singular, s_pa, s_pu);
 }
 
 // Levels below the tail as flat pair programs: workgroup g walks the level
@@ -1147,8 +1153,8 @@ template __global__ void k_factor_tail<double>(DevSym, double *, Ctl *, int, int
 template __global__ void k_factor_tail<float>(DevSym, float *, Ctl *, int, int, int);
 template __global__ void k_tail_factor_ext<double>(DevSym, DevFlat, double *, Ctl *, int, int);
 template __global__ void k_tail_factor_ext<float>(DevSym, DevFlat, float *, Ctl *, int, int);
-template __global__ void k_factor_tail_flat<double>(DevSym, DevFlat, double *, Ctl *, int, int);
-template __global__ void k_factor_tail_flat<float>(DevSym, DevFlat, float *, Ctl *, int, int);
+template __global__ void k_factor_tail_flat<double>(DevSym, DevFlat, double *, Ctl *, int, int, int);
+template __global__ void k_factor_tail_flat<float>(DevSym, DevFlat, float *, Ctl *, int, int, int);
 template __global__ void k_factor_parts_flat<double>(DevSym, DevFlat, double *, Ctl *, int);
 template __global__ void k_factor_parts_flat<float>(DevSym, DevFlat, float *, Ctl *, int);
 template __global__ void k_fwd_parts_lds<double>(DevSym, DevParts, const double *, double *, Ctl *, int);

@@ -116,10 +116,11 @@ struct DevTail {
 
 // A flat pair program on the device (FlatTail on the host): level steps,
 // each its destinations [de_ptr[t], de_ptr[t+1]) and pairs
-// [fl_ptr[t], fl_ptr[t+1]); lv0: the first level of a level-by-level program.
+// [fl_ptr[t], fl_ptr[t+1]) (fl_s: a pair's staged slots in the previous
+// step, or -1); lv0: the first level of a level-by-level program.
 struct DevFlat {
     int lv0, max_pairs, max_col, de_sentinel;
-    const int *fl_ptr, *fl_split, *fl_a, *fl_b, *de_ptr;
+    const int *fl_ptr, *fl_split, *fl_a, *fl_b, *fl_s, *de_ptr;
     const int *de_node, *de_p, *de_q0, *de_ext, *de_lo, *de_hi, *de_dlo, *de_dhi, *de_lr, *st_g;
 };
 
@@ -142,8 +143,15 @@ inline long long part_sweep_lds(int max_nodes, int max_entries, bool backward) {
 // bytes of LDS of the flat factorisation: the pair products of one level
 // (the second product only for column destinations)
 template <typename T>
-inline long long flat_tail_lds(int max_pairs, int max_col) {
+__host__ __device__ inline long long flat_tail_lds(int max_pairs, int max_col) {
     return (long long)sizeof(T) * 4 * ((long long)max_pairs + max_col);
+}
+
+// bytes of LDS that stage the L and U blocks of nd destinations of a flat
+// step for the next one
+template <typename T>
+__host__ __device__ inline long long flat_stage_lds(int nd) {
+    return (long long)sizeof(T) * 8 * nd;
 }
 
 // bytes of dynamic LDS of the tail sweeps: x of the tail nodes (2 doubles

@@ -15,7 +15,7 @@ template <typename T> __global__ void k_jfill(DevNet N, DevState S, DevSym Y, T 
 template <typename T> __global__ void k_factor_level(DevSym Y, T *blk, Ctl *ctl, int l0, int count, int gate);
 template <typename T> __global__ void k_factor_tail(DevSym Y, T *blk, Ctl *ctl, int lv0, int lv1, int gate);
 template <typename T> __global__ void k_tail_factor_ext(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int ne, int gate);
-template <typename T> __global__ void k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int nsteps, int gate);
+template <typename T> __global__ void k_factor_tail_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int nsteps, int gate, int cap);
 template <typename T> __global__ void k_factor_parts_flat(DevSym Y, DevFlat F, T *blk, Ctl *ctl, int gate);
 template <typename T> __global__ void k_factor_parts(DevSym Y, T *blk, Ctl *ctl, int gate);
 template <typename T> __global__ void k_chain_schur(DevSym Y, T *blk, Ctl *ctl, int gate);

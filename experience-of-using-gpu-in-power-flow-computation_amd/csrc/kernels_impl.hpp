@@ -426,14 +426,24 @@ __device__ __forceinline__ B4<T> sub4(const B4<T> &a, const B4<T> &x) {
 // reads pair 0's slot, a lane without a destination reads the program's
 // sentinel destination (a pivot of node 0 without pairs), and a step past
 // t1 is empty.  s_pa and s_pu hold F.max_pairs and F.max_col blocks.
+// With staging (s_st: 2 cap blocks, cap <= kFlatStageMax; cap = 0: none),
+// a step that follows one of at most cap destinations does not wait for
+// its pairs' blocks in A: B of the previous step leaves each destination's
+// L and U blocks in s_st as well, the pairs whose source is a node of that
+// step (FlatTail::fl_s) read them there, and every other pair's blocks
+// (final two steps before) are requested at the start of the previous
+// step's B, whose work covers their latency.  (The step's destinations'
+// own blocks are still requested in A: B needs them last.)  The arithmetic
+// is the same either way.
 template <typename T, int R>
 __device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFlat &F, T *blk, int t0, int t1, int *bad, T *s_pa,
-                                   T *s_pu) {
+                                   T *s_pu, T *s_st = nullptr, int cap = 0) {
     if (t0 >= t1) return;
     const int L0 = Y.n, U0 = Y.n + Y.nnzL, LR0 = Y.n + 2 * Y.nnzL, DI = Y.di0;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
     struct Step {
         int base, np, ns, e0, ne, g;
+        bool f;   // the pair blocks come from staging and from requests made during the previous B
     };
     // the program's arrays are constant while it runs: their uniform loads
     // go through the scalar cache
@@ -447,19 +457,21 @@ __device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFla
         S.e0 = ld_const(F.de_ptr, tc);
         S.ne = in ? ld_const(F.de_ptr, tc + 1) - S.e0 : 0;
         S.g = ld_const(F.st_g, tc);
+        S.f = cap > 0 && in && t > t0 && S.e0 - ld_const(F.de_ptr, tc > 0 ? tc - 1 : 0) <= cap;
         return S;
     };
     auto lane_group = [](int g, int x) { return g == 1 ? x : g == 4 ? x >> 2 : x >> 4; };
-    int pa[R], pb[R];
+    int pa[R], pb[R], ps[R];
     auto pairs_of = [&](const Step &S) __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int e = tid + r * bd;
             const bool in = e < S.np;
             const int q = in ? S.base + e : S.base;   // the sentinel pair when the step is empty
-            const int x = F.fl_a[q], y = F.fl_b[q];
+            const int x = F.fl_a[q], y = F.fl_b[q], z = F.fl_s[q];
             pa[r] = in ? x : 0;
             pb[r] = in ? y : 0;
+            ps[r] = in ? z : -1;
         }
     };
     struct Idx {
@@ -484,7 +496,7 @@ __device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFla
     // pivot's, u: the U block's) in order, two at a time with their loads
     // in flight together; the group then adds its lanes' sums as a tree and
     // its first lane stores the pivot inverse or the scaled L and the U block
-    auto dest_b = [&](int g, int ns, const Idx &D, const Val &V) __attribute__((always_inline)) {
+    auto dest_b = [&](int g, int ns, const Idx &D, const Val &V, int slot, bool stage) __attribute__((always_inline)) {
         const int lane = tid & (g - 1);
         const bool col = D.p >= 0;
         const int na = D.hi - D.lo, nd = col ? D.dhi - D.dlo : 0;
@@ -529,25 +541,44 @@ __device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFla
             if (sing) atomicOr(bad, 1);
             store_b4(blk, DI + D.j, Pi);
         } else {
-            const B4<T> lb = mul(sub4(V.v0, a), Pi);
+            const B4<T> lb = mul(sub4(V.v0, a), Pi), ub = sub4(V.v1, u);
             store_b4(blk, L0 + D.p, lb);
             store_b4(blk, LR0 + D.lr, lb);
-            store_b4(blk, U0 + D.p, sub4(V.v1, u));
+            store_b4(blk, U0 + D.p, ub);
+            if (stage) {
+                store_b4(s_st, 2 * slot, lb);
+                store_b4(s_st, 2 * slot + 1, ub);
+            }
         }
     };
     Step S = step_at(t0), S1 = step_at(t0 + 1);
     pairs_of(S);
     Idx nx = idx_of(first_dest(S));   // this thread's first destination of the coming step
-    for (int t = t0; t < t1; t++) {
-        const Idx mi = nx;
-        const Val mv = val_of(mi);
-        B4<T> la[R], ub[R], lb[R], ua[R];
+    B4<T> la[R], ub[R], lb[R], ua[R];   // this step's pair blocks
+    auto request = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             la[r] = load_b4(blk, L0 + pa[r]);
             ub[r] = load_b4(blk, U0 + pb[r]);
             lb[r] = load_b4(blk, L0 + pb[r]);
             ua[r] = load_b4(blk, U0 + pa[r]);
+        }
+    };
+    for (int t = t0; t < t1; t++) {
+        const Idx mi = nx;
+        const Val mv = val_of(mi);
+        if (S.f) {   // requested during the last B; the pairs from its nodes from staging
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (ps[r] >= 0) {
+                    const int sa = ps[r] & 0xffff, sb = ps[r] >> 16;
+                    la[r] = load_b4(s_st, 2 * sa);
+                    ua[r] = load_b4(s_st, 2 * sa + 1);
+                    lb[r] = load_b4(s_st, 2 * sb);
+                    ub[r] = load_b4(s_st, 2 * sb + 1);
+                }
+        } else {
+            request();
         }
         pairs_of(S1);
         nx = idx_of(first_dest(S1));
@@ -566,11 +597,12 @@ __device__ __forceinline__ void flat_factor_levels(const DevSym &Y, const DevFla
         }
         __syncthreads();
         {   // B: the thread's first destination from registers, further ones (wide steps) loaded here
+            if (S1.f) request();   // the next step's pair blocks that are final already
             const int grp = lane_group(S.g, tid), ng = lane_group(S.g, bd);
-            if (grp < S.ne) dest_b(S.g, S.ns, mi, mv);
+            if (grp < S.ne) dest_b(S.g, S.ns, mi, mv, grp, S1.f);
             for (int e = grp + ng; e < S.ne; e += ng) {
                 const Idx D = idx_of(S.e0 + e);
-                dest_b(S.g, S.ns, D, val_of(D));
+                dest_b(S.g, S.ns, D, val_of(D), e, S1.f);
             }
         }
         __syncthreads();

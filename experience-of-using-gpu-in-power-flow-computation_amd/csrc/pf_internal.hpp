@@ -27,6 +27,9 @@ namespace pf {
 // one block: FP32 factor (kChainMax) and FP64 factor (kChainMaxF64), as many
 // as its register tiles hold
 constexpr int kChainMax = 128;
+// most destinations of a flat step whose blocks the next step may read from
+// LDS staging (slots are 15-bit in FlatTail::fl_s)
+constexpr int kFlatStageMax = 32767;
 constexpr int kChainMaxF64 = 101;
 // widest elimination-tree level the dense top block may take
 constexpr int kTopWidth = 1;
@@ -157,7 +160,12 @@ TailSweeps tail_sweeps(const Symbolic &s);
 // st_g[t] lanes (1, 4 or 16, a function of the step alone: see
 // flat_group_width): lane i adds products lo + i, lo + i + G, ... in order
 // and the lanes' sums are added as a balanced tree in lane order
-// (flat_group_sum restates it).  max_pairs / max_col bound one step.  The
+// (flat_group_sum restates it).  A pair whose source node belongs to the
+// previous step (its blocks were formed there) has fl_s = sa | sb << 16,
+// the indices of its two positions' destinations in that step's list, so
+// a kernel can read them from where that step staged them; every other
+// pair has fl_s = -1 (also a fresh pair after a step of more than
+// kFlatStageMax destinations).  max_pairs / max_col bound one step.  The
 // pair and destination lists end with one sentinel entry past the program
 // (pair 0; a pivot of node 0 without pairs), which lanes without a pair or
 // a destination read when they prefetch; de_ptr.back() is its index.
@@ -170,7 +178,7 @@ TailSweeps tail_sweeps(const Symbolic &s);
 //   = the nodes of group g on level l (every pair is inside the group).
 struct FlatTail {
     int lv0 = 0;
-    std::vector<int> fl_ptr, fl_split, fl_a, fl_b, de_ptr;
+    std::vector<int> fl_ptr, fl_split, fl_a, fl_b, fl_s, de_ptr;
     std::vector<int> de_node, de_p, de_q0, de_ext, de_lo, de_hi, de_dlo, de_dhi, de_lr, st_g;
     int max_pairs = 0, max_col = 0;
 };

@@ -72,6 +72,7 @@ struct pf_solver {
     int psw_lds[2][2] = {{0, 0}, {0, 0}};
     int flat_ne = 0;
     int flat_lds[2] = {0, 0}, parts_lds[2] = {0, 0};
+    int flat_cap = 0;   // FP32 tail: most destinations of a step whose blocks the next step reads from LDS
     int tail_mode[2] = {0, 0};
     int tail_lds_f[2] = {0, 0}, tail_lds_b[2] = {0, 0};
     double *blk64 = nullptr;
@@ -443,6 +444,7 @@ pf_status pf_solver::upload_flat(const FlatTail &ft, DevFlat &F) {
     PFCHK(upload((int **)&F.fl_split, ft.fl_split));
     PFCHK(upload((int **)&F.fl_a, ft.fl_a));
     PFCHK(upload((int **)&F.fl_b, ft.fl_b));
+    PFCHK(upload((int **)&F.fl_s, ft.fl_s));
     PFCHK(upload((int **)&F.de_ptr, ft.de_ptr));
     PFCHK(upload((int **)&F.de_node, ft.de_node));
     PFCHK(upload((int **)&F.de_p, ft.de_p));
@@ -474,8 +476,14 @@ pf_status pf_solver::setup_flat_tail() {
     PFCHK(upload_flat(ft, dfl));
     PFCHK(allow((const void *)k_factor_tail_flat<double>,
                 std::max<long long>(flat_tail_lds<double>(ft.max_pairs, ft.max_col), 16), flat_lds[0]));
-    PFCHK(allow((const void *)k_factor_tail_flat<float>,
-                std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16), flat_lds[1]));
+    {   // FP32: the products, then the staged blocks of as many destinations as fit
+        const long long pf = std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16);
+        int max_ne = 0;
+        for (size_t t = 0; t + 1 < ft.de_ptr.size(); t++) max_ne = std::max(max_ne, ft.de_ptr[t + 1] - ft.de_ptr[t]);
+        flat_cap = (int)std::max<long long>(
+            0, std::min<long long>({(long long)max_ne, (long long)kFlatStageMax, (cap - pf) / flat_stage_lds<float>(1)}));
+        PFCHK(allow((const void *)k_factor_tail_flat<float>, pf + flat_stage_lds<float>(flat_cap), flat_lds[1]));
+    }
     if (sym.n_parts > 0) {
         const PartSweeps ps = part_sweeps(sym);
         dps.max_nodes = ps.max_nodes;
@@ -529,7 +537,8 @@ pf_status pf_solver::enqueue_lu(T *blk, int gate) {
     const int fl = sym.tail_start < sym.nlev_s ? flat_lds[sizeof(T) == 4 ? 1 : 0] : 0;
     if (fl) {   // narrow levels below the chain: pairs from below the tail across the GPU first
         k_tail_factor_ext<T><<<(flat_ne + 31) / 32, 256, 0, stream>>>(dy, dfl, blk, ctl, flat_ne, gate);
-        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.nlev_s - sym.tail_start, gate);
+        k_factor_tail_flat<T><<<1, 1024, fl, stream>>>(dy, dfl, blk, ctl, sym.nlev_s - sym.tail_start, gate,
+                                                       sizeof(T) == 4 ? flat_cap : 0);
     } else if (sym.tail_start < sym.nlev_s) {
         k_factor_tail<T><<<1, 1024, 0, stream>>>(dy, blk, ctl, sym.tail_start, sym.nlev_s, gate);
     }

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <queue>
+#include <stdexcept>
 
 #include "pf_internal.hpp"
 
@@ -457,7 +458,7 @@ namespace {
 // one level step of a flat program: the destinations of the given nodes,
 // the pivots' pairs first (pairs from sources k < t0 left out)
 void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0, const std::vector<int> &pcol,
-               std::vector<int> &dlo, std::vector<int> &dhi) {
+               std::vector<int> &dlo, std::vector<int> &dhi, std::vector<int> &pos_de) {
     const int base = f.fl_ptr.back();
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1) f.fl_split.push_back((int)f.fl_a.size() - base);
@@ -468,6 +469,7 @@ void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0,
                 if ((t == 0) != (pass == 0)) continue;
                 int q = s.fc_ptr[d];
                 while (q < s.fc_ptr[d + 1] && pcol[s.fc_b[q]] < t0) q++;   // pairs ascend in k
+                if (t > 0) pos_de[s.colptr[j] + t - 1] = (int)f.de_node.size();
                 f.de_node.push_back(j);
                 f.de_p.push_back(t == 0 ? -1 : s.colptr[j] + t - 1);
                 f.de_lr.push_back(t == 0 ? 0 : s.lr_pos[s.colptr[j] + t - 1]);
@@ -490,6 +492,15 @@ void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0,
     }
     const int np = (int)f.fl_a.size() - base;
     const int E0 = f.de_ptr.back(), E1 = (int)f.de_node.size();
+    // pairs whose source is a node of the previous step: the slots of their
+    // two positions among that step's destinations
+    const int P0 = f.de_ptr.size() >= 2 ? f.de_ptr[f.de_ptr.size() - 2] : E0;
+    for (int q = base; q < base + np; q++) {
+        const int ea = pos_de[f.fl_a[q]], eb = pos_de[f.fl_b[q]];
+        const bool fresh = ea >= P0 && ea < E0;
+        if (fresh != (eb >= P0 && eb < E0)) throw std::logic_error("flat_step: pair positions in different steps");
+        f.fl_s.push_back(fresh && E0 - P0 <= kFlatStageMax ? (ea - P0) | (eb - P0) << 16 : -1);
+    }
     int maxpd = 0;
     for (int E = E0; E < E1; E++) maxpd = std::max(maxpd, f.de_hi[E] - f.de_lo[E]);
     f.st_g.push_back(flat_group_width(E1 - E0, maxpd));
@@ -502,6 +513,7 @@ void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0,
 void flat_seal(FlatTail &f) {
     f.fl_a.push_back(0);
     f.fl_b.push_back(0);
+    f.fl_s.push_back(-1);
     f.de_node.push_back(0);
     f.de_p.push_back(-1);
     f.de_q0.push_back(0);
@@ -527,13 +539,13 @@ FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below) {
     f.lv0 = lv0;
     const int t0 = split_below ? s.lvl_ptr[lv0] : 0;
     const std::vector<int> pcol = position_columns(s);
-    std::vector<int> dlo(s.n), dhi(s.n);
+    std::vector<int> dlo(s.n), dhi(s.n), pos_de(s.nnzL, -1);
     f.fl_ptr.assign(1, 0);
     f.de_ptr.assign(1, 0);
     for (int l = lv0; l < s.nlev_s; l++) {   // nodes are numbered level by level
         std::vector<int> nodes;
         for (int j = s.lvl_ptr[l]; j < s.lvl_ptr[l + 1]; j++) nodes.push_back(j);
-        flat_step(s, f, nodes.data(), (int)nodes.size(), t0, pcol, dlo, dhi);
+        flat_step(s, f, nodes.data(), (int)nodes.size(), t0, pcol, dlo, dhi, pos_de);
     }
     flat_seal(f);
     return f;
@@ -543,13 +555,13 @@ FlatTail flat_parts(const Symbolic &s) {
     FlatTail f;
     const int T = s.tail_start;
     const std::vector<int> pcol = position_columns(s);
-    std::vector<int> dlo(s.n), dhi(s.n);
+    std::vector<int> dlo(s.n), dhi(s.n), pos_de(s.nnzL, -1);
     f.fl_ptr.assign(1, 0);
     f.de_ptr.assign(1, 0);
     for (int g = 0; g < s.n_parts; g++)
         for (int l = 0; l < T; l++) {
             const int a = s.part_ptr[(size_t)g * T + l], b = s.part_ptr[(size_t)g * T + l + 1];
-            flat_step(s, f, s.part_nodes.data() + a, b - a, 0, pcol, dlo, dhi);
+            flat_step(s, f, s.part_nodes.data() + a, b - a, 0, pcol, dlo, dhi, pos_de);
         }
     flat_seal(f);
     return f;
@@ -724,6 +736,9 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
     auto U = [&](int p) { return blk + 4 * (size_t)(U0 + p); };
     auto run = [&](const FlatTail &f) -> bool {
         const int nsteps = (int)f.fl_ptr.size() - 1;
+        // the previous step's destinations' L and U blocks by their index in
+        // its list: the pairs with a source there (fl_s >= 0) read them here
+        std::vector<double> stage, next;
         for (int t = 0; t < nsteps; t++) {
             const int base = f.fl_ptr[t], np = f.fl_ptr[t + 1] - base, ns = f.fl_split[t], G = f.st_g[t];
             auto sum = [G](const std::vector<double> &v, int lo, int hi, double *a) {
@@ -731,9 +746,21 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
             };
             std::vector<double> pa(4 * (size_t)np), pu(4 * (size_t)std::max(np - ns, 0));
             for (int e = 0; e < np; e++) {   // every product from earlier steps' blocks
-                mul2(&pa[4 * (size_t)e], L(f.fl_a[base + e]), U(f.fl_b[base + e]));
-                if (e >= ns) mul2(&pu[4 * (size_t)(e - ns)], L(f.fl_b[base + e]), U(f.fl_a[base + e]));
+                const int fs = f.fl_s[base + e];
+                const double *la = L(f.fl_a[base + e]), *ua = U(f.fl_a[base + e]);
+                const double *lb = L(f.fl_b[base + e]), *ub = U(f.fl_b[base + e]);
+                if (fs >= 0) {
+                    const size_t sa = (size_t)(fs & 0xffff), sb = (size_t)(fs >> 16);
+                    if (8 * std::max(sa, sb) + 8 > stage.size()) return false;
+                    la = &stage[8 * sa];
+                    ua = &stage[8 * sa + 4];
+                    lb = &stage[8 * sb];
+                    ub = &stage[8 * sb + 4];
+                }
+                mul2(&pa[4 * (size_t)e], la, ub);
+                if (e >= ns) mul2(&pu[4 * (size_t)(e - ns)], lb, ua);
             }
+            next.assign(8 * (size_t)(f.de_ptr[t + 1] - f.de_ptr[t]), std::nan(""));
             for (int E = f.de_ptr[t]; E < f.de_ptr[t + 1]; E++) {
                 const int j = f.de_node[E], p = f.de_p[E];
                 double a[4], P[4];
@@ -757,8 +784,11 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
                     }
                     mul2(lb, Lb, Pi);
                     std::copy(lb, lb + 4, L(p));
+                    std::copy(lb, lb + 4, &next[8 * (size_t)(E - f.de_ptr[t])]);
+                    std::copy(U(p), U(p) + 4, &next[8 * (size_t)(E - f.de_ptr[t]) + 4]);
                 }
             }
+            stage.swap(next);
         }
         return true;
     };

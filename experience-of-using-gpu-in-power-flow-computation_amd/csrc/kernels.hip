@@ -406,12 +406,13 @@ __global__ void __launch_bounds__(256) k_chain_schur(DevSym Y, T *blk, Ctl *ctl,
     chain_schur<T, 16, 4>(Y, blk, (int)blockIdx.x * 16, (int)gridDim.x * 16);
 }
 
-// Gauss-Jordan inverse of the chain's Schur complement, in one workgroup.
+// Gauss-Jordan inverse of the chain's Schur complement, in one workgroup (in FP32
+// with the fused update: this solver's FP32 factor is refined in FP64).
 template <typename T>
 __global__ void __launch_bounds__(ChainTiles<T>::threads) k_chain_invert(DevSym Y, T *blk, Ctl *ctl, int gate) {
     if (gate && !ctl->step) return;
     __shared__ B4<T> lds[4 * kChainMax + 2];
-    chain_invert<T, ChainTiles<T>::tr, ChainTiles<T>::tc>(Y, blk, &ctl->singular, lds);
+    chain_invert<T, ChainTiles<T>::tr, ChainTiles<T>::tc, sizeof(T) == 4>(Y, blk, &ctl->singular, lds);
 }
 
 // ----------------------------------------------------- triangular solves

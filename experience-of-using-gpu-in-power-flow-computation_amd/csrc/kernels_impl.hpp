@@ -38,6 +38,17 @@ __device__ __forceinline__ void mms(B4<T> &c, const B4<T> &a, const B4<T> &b) {
     c.d -= a.c * b.b + a.d * b.d;
 }
 
+// c -= a * b as two fused multiply-adds per entry, in the order of the
+// terms (two roundings per entry instead of three, two instructions instead
+// of three)
+template <typename T>
+__device__ __forceinline__ void mms_fused(B4<T> &c, const B4<T> &a, const B4<T> &b) {
+    c.a = fma(-a.b, b.c, fma(-a.a, b.a, c.a));
+    c.b = fma(-a.b, b.d, fma(-a.a, b.b, c.b));
+    c.c = fma(-a.d, b.c, fma(-a.c, b.a, c.c));
+    c.d = fma(-a.d, b.d, fma(-a.c, b.b, c.d));
+}
+
 template <typename T>
 __device__ __forceinline__ B4<T> mul(const B4<T> &a, const B4<T> &b) {
     return B4<T>{a.a * b.a + a.b * b.c, a.a * b.b + a.b * b.d, a.c * b.a + a.d * b.c,
@@ -712,8 +723,10 @@ __device__ __forceinline__ void gj_update(B4<T> &s, int i, int j, int k, const B
 // column k then overwrite those blocks with P S_kj, -S_ik P and P, so the
 // update loop carries no per-block branches (blocks past m are padding,
 // updated but never stored).  Both forms give the same values.
+// FUSED (FP32): the general update as mms_fused, which rounds differently
+// from mms; the memory fallback and the batch kernel keep mms.
 // The caller orders it after chain_schur and synchronises after.
-template <typename T, int TR, int TC>
+template <typename T, int TR, int TC, bool FUSED = false>
 __device__ __forceinline__ void chain_invert(const DevSym &Y, T *blk, int *bad, B4<T> *lds) {
     const int m = Y.ch_m, SB = Y.n + 3 * Y.nnzL;
     const int tid = (int)threadIdx.x;
@@ -764,7 +777,10 @@ __device__ __forceinline__ void chain_invert(const DevSym &Y, T *blk, int *bad, 
 #pragma unroll
             for (int a = 0; a < TR; a++)
 #pragma unroll
-                for (int c = 0; c < TC; c++) mms(reg[a][c], ci[a], pr[c]);
+                for (int c = 0; c < TC; c++) {
+                    if constexpr (FUSED) mms_fused(reg[a][c], ci[a], pr[c]);
+                    else mms(reg[a][c], ci[a], pr[c]);
+                }
             if ((unsigned)ar < (unsigned)TR) {   // row k: P S_kj
 #pragma unroll
                 for (int a = 0; a < TR; a++)

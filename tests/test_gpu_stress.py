@@ -10,7 +10,11 @@ refined step leaves the Newton path unchanged up to the nose), and fail on
 the others (the batch kernel stops a blown-up scenario early, the oracle runs
 it to the limit).
 The factors were chosen on the oracle: the last ones below the nose in steps
-of 0.01-0.05 (min |V| 0.55-0.66 pu there), and one past it."""
+of 0.01-0.05 (min |V| 0.55-0.67 pu there), and one past it.  At 15,104 buses
+the dense chain is a 202 x 202 FP32 Gauss-Jordan without pivoting; there the
+flat-start Newton path is erratic near the nose (without Q limits 1.28-1.34
+diverge while 1.36-1.38 converge), and the test takes converging factors on
+both sides of that band and a diverging one inside it."""
 import os
 import sys
 
@@ -25,7 +29,9 @@ pytestmark = pytest.mark.gpu
 LOADS = {
     "c118": {True: ([1.5, 1.6, 1.7], 1.9), False: ([2.0, 2.1, 2.15, 2.18], 2.4)},
     "c1888": {True: ([1.2, 1.24, 1.28], 1.6), False: ([1.5, 1.6, 1.62, 1.64], 1.8)},
+    "c15104": {True: ([1.1, 1.15], 1.19), False: ([1.2, 1.24, 1.38], 1.3)},
 }
+_REF = {}   # (network, limits): the oracle's solves, shared by the parametrisations
 
 
 def _say(*a):
@@ -44,10 +50,12 @@ def _injections(net, lam):
 
 def _reference(oracle, p, net, name, lim):
     conv, fail = LOADS[name][lim]
-    oc = oracle.Case(p)
     lams = conv + [fail]
-    ref = [oc.solve(tol=1e-8, max_iter=60, lim=lim, reuse_symbolic=True, **dict(zip(
-        ("Pliq", "Qliq", "Qload"), _injections(net, lam)))) for lam in lams]
+    if (name, lim) not in _REF:
+        oc = oracle.Case(p)
+        _REF[name, lim] = [oc.solve(tol=1e-8, max_iter=60, lim=lim, reuse_symbolic=True, **dict(zip(
+            ("Pliq", "Qliq", "Qload"), _injections(net, lam)))) for lam in lams]
+    ref = _REF[name, lim]
     # the chosen factors are what they claim to be
     assert [r["converged"] for r in ref] == [True] * len(conv) + [False]
     return lams, ref
@@ -66,7 +74,7 @@ def _check(name, lam, r, conv, V, th, lq, it):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("lim", [True, False])
-@pytest.mark.parametrize("name", ["c118", "c1888"])
+@pytest.mark.parametrize("name", ["c118", "c1888", "c15104"])
 def test_batch_near_collapse(pf, oracle, cases, name, lim, precision):
     p = _path(cases, name)
     net = pf.Network(p)
@@ -90,7 +98,7 @@ def test_batch_near_collapse(pf, oracle, cases, name, lim, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("lim", [True, False])
-@pytest.mark.parametrize("name", ["c118", "c1888"])
+@pytest.mark.parametrize("name", ["c118", "c1888", "c15104"])
 def test_solver_near_collapse(pf, oracle, cases, name, lim, precision):
     p = _path(cases, name)
     net = pf.Network(p)

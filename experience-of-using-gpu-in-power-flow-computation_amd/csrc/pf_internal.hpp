@@ -181,6 +181,7 @@ struct FlatTail {
     std::vector<int> fl_ptr, fl_split, fl_a, fl_b, fl_s, de_ptr;
     std::vector<int> de_node, de_p, de_q0, de_ext, de_lo, de_hi, de_dlo, de_dhi, de_lr, st_g;
     int max_pairs = 0, max_col = 0;
+    bool stage_ok = true;   // fl_s is usable (a pair's two positions always share a step)
 };
 FlatTail flat_tail(const Symbolic &s, int lv0, bool split_below);
 FlatTail flat_parts(const Symbolic &s);

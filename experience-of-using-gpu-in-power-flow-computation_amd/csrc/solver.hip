@@ -480,7 +480,7 @@ pf_status pf_solver::setup_flat_tail() {
         const long long pf = std::max<long long>(flat_tail_lds<float>(ft.max_pairs, ft.max_col), 16);
         int max_ne = 0;
         for (size_t t = 0; t + 1 < ft.de_ptr.size(); t++) max_ne = std::max(max_ne, ft.de_ptr[t + 1] - ft.de_ptr[t]);
-        flat_cap = (int)std::max<long long>(
+        flat_cap = !ft.stage_ok ? 0 : (int)std::max<long long>(
             0, std::min<long long>({(long long)max_ne, (long long)kFlatStageMax, (cap - pf) / flat_stage_lds<float>(1)}));
         PFCHK(allow((const void *)k_factor_tail_flat<float>, pf + flat_stage_lds<float>(flat_cap), flat_lds[1]));
     }

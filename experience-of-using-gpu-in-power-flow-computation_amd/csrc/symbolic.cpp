@@ -20,7 +20,6 @@
 #include <cmath>
 #include <cstdint>
 #include <queue>
-#include <stdexcept>
 
 #include "pf_internal.hpp"
 
@@ -498,7 +497,7 @@ void flat_step(const Symbolic &s, FlatTail &f, const int *nodes, int nn, int t0,
     for (int q = base; q < base + np; q++) {
         const int ea = pos_de[f.fl_a[q]], eb = pos_de[f.fl_b[q]];
         const bool fresh = ea >= P0 && ea < E0;
-        if (fresh != (eb >= P0 && eb < E0)) throw std::logic_error("flat_step: pair positions in different steps");
+        if (fresh != (eb >= P0 && eb < E0)) f.stage_ok = false;   // both lie in column k: cannot happen
         f.fl_s.push_back(fresh && E0 - P0 <= kFlatStageMax ? (ea - P0) | (eb - P0) << 16 : -1);
     }
     int maxpd = 0;
@@ -749,7 +748,7 @@ bool host_flat_factor(const Symbolic &s, const FlatTail &parts, const FlatTail &
                 const int fs = f.fl_s[base + e];
                 const double *la = L(f.fl_a[base + e]), *ua = U(f.fl_a[base + e]);
                 const double *lb = L(f.fl_b[base + e]), *ub = U(f.fl_b[base + e]);
-                if (fs >= 0) {
+                if (fs >= 0 && f.stage_ok) {
                     const size_t sa = (size_t)(fs & 0xffff), sb = (size_t)(fs >> 16);
                     if (8 * std::max(sa, sb) + 8 > stage.size()) return false;
                     la = &stage[8 * sa];
